@@ -1,0 +1,220 @@
+/*
+ * pvlbm.h — C ABI of the MI355X-native lattice Boltzmann engine for vessel blood flow.
+ *
+ * The reference solver (Resacd/PV_Dec_Original) exposes no FFI: its interface is four C++
+ * constructors called from main() (LBM.cpp:69-72).  Each entry point below replaces one step of
+ * that object pipeline; the comment on each names the reference code it stands in for.
+ *
+ * Conventions
+ *   - Every call returns an int status: PVLBM_OK (0) or a positive error code.  The text of the
+ *     last error is returned by pvlbm_last_error(handle) (or pvlbm_last_error(NULL) for calls that
+ *     take no handle).  No C++ exception crosses this boundary.
+ *   - Host pointers are owned by the caller and are copied synchronously; every call has finished
+ *     its device work when it returns.  A handle owns all of its device memory and its own HIP
+ *     stream; use one handle per host thread.
+ *   - Box arrays are dense over the grid with x fastest: index = i + nx*(j + ny*k), the node at
+ *     (origin_x + i*h, origin_y + j*h, origin_z + k*h) (ComplexGeometry.cpp:337-340).
+ *   - Triangles are 12 doubles each: v1, v2, v3, then the facet normal exactly as the STL file
+ *     gives it (Struct.h:117-125 order; the normal is not recomputed, GeometryReader.cpp:249).
+ */
+#ifndef PVLBM_H
+#define PVLBM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pvlbm_sim* pvlbm_handle;
+
+enum pvlbm_status_code {
+    PVLBM_OK = 0,
+    PVLBM_EINVAL = 1, /* bad argument */
+    PVLBM_EHIP = 2,   /* HIP runtime error (no GPU, launch failure, ...) */
+    PVLBM_ENOMEM = 3, /* device or host allocation failed */
+    PVLBM_EIO = 4,    /* file could not be read */
+    PVLBM_ESTATE = 5  /* call not valid in the handle's current state */
+};
+
+/* Population storage of a handle.  FP64 is the plain path (parity with the reference).  FP16S and
+ * BF16S store the shifted populations f_i - w_i in 16 bits and collide in FP32; global sums stay
+ * in FP64.  The shifted forms have no reference counterpart. */
+enum pvlbm_storage { PVLBM_FP64 = 0, PVLBM_FP16S = 1, PVLBM_BF16S = 2 };
+
+/* Roles of an open boundary (ComplexGeometry.h:46-49: inlet, outlet 1..3). */
+enum pvlbm_role { PVLBM_INLET = 0, PVLBM_OUTLET1 = 1, PVLBM_OUTLET2 = 2, PVLBM_OUTLET3 = 3 };
+
+/* Closure applied on a face.  INLET_XMIN and OUTLET_YMAX / OUTLET_ZMIN are the reference's
+ * formulas (Boundary.cpp:129-157 and :87-123; Simulation.cpp:642-695 and :710-836).  OUTLET_XMAX
+ * has no reference formula: for D3Q19 it is OUTLET_YMAX under the x<->y index permutation, for
+ * D3Q27 the mirror image of the D3Q27 inlet with the density prescribed. */
+enum pvlbm_closure {
+    PVLBM_INLET_XMIN = 0,
+    PVLBM_OUTLET_YMAX = 1,
+    PVLBM_OUTLET_ZMIN = 2,
+    PVLBM_OUTLET_XMAX = 3
+};
+
+/* Node kinds reported by pvlbm_get_kinds. */
+enum pvlbm_kind {
+    PVLBM_SOLID = 0,  /* not a lattice node (NodeType::NotFluid) */
+    PVLBM_FLUID = 1,  /* interior fluid node */
+    PVLBM_NTW = 2,    /* interior fluid node next to a wall node (Subdomain.cpp:201-218) */
+    PVLBM_KINLET = 3, /* inlet boundary node */
+    PVLBM_KOUTLET = 4,/* outlet boundary node (any of outlets 1..3) */
+    PVLBM_WALL = 5    /* wall boundary node */
+};
+
+/* Lists that pvlbm_count / pvlbm_list report. */
+enum pvlbm_list_id {
+    PVLBM_LIST_FLUID = 0,   /* every lattice node */
+    PVLBM_LIST_INLET = 1,
+    PVLBM_LIST_OUTLET1 = 2,
+    PVLBM_LIST_OUTLET2 = 3,
+    PVLBM_LIST_OUTLET3 = 4,
+    PVLBM_LIST_WALL = 5,
+    PVLBM_LIST_NTW = 6,
+    PVLBM_LIST_TILES = 7    /* active 64-node tiles of the step kernel */
+};
+
+/* A face specification: the reference hard-codes four planes of PV.stl
+ * (ComplexGeometry.cpp:1378-1408); this generalises each to {axis, plane, tolerance, window}.
+ * A boundary node at (c0, c1, c2) matches when |c[axis] - plane| <= tol and, for the two other
+ * axes a < b in increasing order, lo[0] < c[a] < hi[0] and lo[1] < c[b] < hi[1], compared in
+ * long double as the reference does.  Matching is tried in the order inlet, outlet1, outlet2,
+ * outlet3 (Subdomain.cpp:124-155); a boundary node that matches nothing is a wall node. */
+typedef struct pvlbm_face {
+    int32_t role;    /* enum pvlbm_role */
+    int32_t closure; /* enum pvlbm_closure */
+    int32_t axis;    /* 0 = x, 1 = y, 2 = z */
+    int32_t reserved;
+    double plane;
+    double tol;
+    double lo[2];
+    double hi[2];
+    double rho;      /* prescribed density of a pressure outlet (the reference uses 1.0) */
+} pvlbm_face;
+
+/* A lattice grid: node (i,j,k) sits at origin + (i,j,k)*h. */
+typedef struct pvlbm_grid {
+    double origin[3];
+    int32_t dims[3];
+    int32_t reserved;
+} pvlbm_grid;
+
+/* Text of the last error.  h == NULL gives the calling thread's last handle-less error. */
+const char* pvlbm_last_error(pvlbm_handle h);
+
+/* Number of HIP devices visible (0 when none).  No reference counterpart. */
+int pvlbm_device_count(int* count);
+
+/* Parse an ASCII STL file.  tri == NULL returns only the count.  Malformed facets are skipped
+ * silently, as the reference does.  Replaces Reader::readSTLASCII (GeometryReader.cpp:33-117). */
+int pvlbm_read_stl_ascii(const char* path, double* tri, int64_t capacity, int64_t* ntri);
+
+/* Bounding box {min_x,max_x,min_y,max_y,min_z,max_z} by the reference rule: x over all three
+ * vertices, y and z over vertices 1 and 2 only.  Replaces the bbox part of
+ * Geometry::GetSubBoundingBox (ComplexGeometry.cpp:142-154). */
+int pvlbm_bbox(const double* tri, int64_t ntri, double bbox[6]);
+
+/* The reference grid of a bbox: origin = min corner, nx = ceil(dx/h)+1, ny = ceil(dy/h)+1,
+ * nz = ceil(dz/h) (ComplexGeometry.cpp:163-165). */
+int pvlbm_grid_from_bbox(const double bbox[6], double h, pvlbm_grid* grid);
+
+/* Ray-cast voxelisation on the GPU: a node is fluid when the +z ray from (x, y, z + h/2) crosses
+ * the mesh an odd number of times, with every floating-point step of the reference
+ * (Geometry::labelPoints and helpers, ComplexGeometry.cpp:620-674, 803-1053).  grid == NULL uses
+ * the reference grid of the mesh bbox; otherwise the given grid is used and the ray's bbox is the
+ * union of the mesh bbox and the grid.  grid_out receives the grid used.  fluid_out (may be NULL
+ * to query the grid) receives nx*ny*nz bytes, 1 = fluid, 0 = not fluid.  Replaces
+ * GetSubBoundingBox + generatePointsInSubBox + labelPoints of the Geometry constructor
+ * (ComplexGeometry.cpp:8-22). */
+int pvlbm_voxelize(const double* tri, int64_t ntri, double h, const pvlbm_grid* grid,
+                   pvlbm_grid* grid_out, uint8_t* fluid_out);
+
+/* Build a device-resident simulation from voxel flags: boundary detection over the Q
+ * neighbours (ComplexGeometry.cpp:1259-1300), face classification (:1369-1429), node kinds
+ * (Subdomain.cpp:91-169), neighbour links (:173-197), next-to-wall set (:201-218), area/d^2
+ * weighted normals over all triangles (:224-370), f = w initialisation (Subdomain.cpp:549-564,
+ * Node.cpp:30-37) and the first collision.  lattice is 19 or 27.  tau and dt play the roles of
+ * Simulation(SubDomain&, tau, pressure, del_t) (Simulation.h:26); h enters the WSS constant as
+ * the reference's spacing (Simulation.cpp:880).  Replaces SubDomain::SubDomain and the
+ * set-up half of Simulation::Simulation. */
+int pvlbm_create(const uint8_t* fluid, const pvlbm_grid* grid, double h, int lattice, int storage,
+                 const pvlbm_face* faces, int nfaces, const double* tri, int64_t ntri, double tau,
+                 double dt, pvlbm_handle* out);
+
+/* Change the relaxation time (Simulation::SetRelaxation, Simulation.cpp:867-869). */
+int pvlbm_set_tau(pvlbm_handle h, double tau);
+
+/* Change the time step del_t that enters the WSS constant 1060*dt^2/(h^2*1e-6)
+ * (Simulation::m_dt, Simulation.cpp:10 and :880). */
+int pvlbm_set_dt(pvlbm_handle h, double dt);
+
+/* Size and members (box indices) of a node list. */
+int pvlbm_count(pvlbm_handle h, int list, int64_t* n);
+int pvlbm_list(pvlbm_handle h, int list, int64_t* idx);
+
+/* Node kind of every box point (enum pvlbm_kind), nx*ny*nz bytes. */
+int pvlbm_get_kinds(pvlbm_handle h, uint8_t* kinds);
+
+/* Inlet velocity of each inlet-list node; the x-velocity applied at step s is
+ * inlet_scale[s] * u[node].  Default 1.0 everywhere, which makes inlet_scale the reference's
+ * uniform InletVelocity_ (Simulation.cpp:580, Boundary.cpp:131-133). */
+int pvlbm_set_inlet_profile(pvlbm_handle h, const double* u);
+
+/* Restart from f = f_eq(rho, u) (warm start; no reference counterpart).  rho and u are dense box
+ * arrays (u as three planes x, y, z); rho == NULL means rest (f = w, the reference start,
+ * Node.cpp:30-37).  Resets the step counter, the WSS accumulators and the stale populations. */
+int pvlbm_init_fields(pvlbm_handle h, const double* rho, const double* u);
+
+/* Every how many iterations WSS and its accumulators are evaluated (reference: every iteration,
+ * Simulation.cpp:585-586).  Default 1. */
+int pvlbm_set_wss_every(pvlbm_handle h, int every);
+
+/* Run n iterations of Simulation::Iterate's body (Simulation.cpp:577-586: collide, stream, inlet,
+ * outlets, walls, WSS).  inlet_scale holds n values (NULL = all 1.0). */
+int pvlbm_step(pvlbm_handle h, int n, const double* inlet_scale);
+
+/* As pvlbm_step, also returning the elapsed milliseconds of the launches measured with HIP events
+ * on the handle's stream. */
+int pvlbm_step_timed(pvlbm_handle h, int n, const double* inlet_scale, double* ms);
+
+/* Iterations completed so far. */
+int pvlbm_iteration(pvlbm_handle h, int64_t* it);
+
+/* Density and velocity of the current iteration n, computed as Simulation::Collision does from
+ * the populations after iteration n-1 (Simulation.cpp:52-105).  Dense box arrays; u holds three
+ * planes.  Solid points get 0.  Either pointer may be NULL. */
+int pvlbm_get_macros(pvlbm_handle h, double* rho, double* u);
+
+/* Per next-to-wall node, in list order: x, y, z, WSS, OSI, TAWSS — the columns of
+ * FlowHemodynamics/<iter>_FlowHemodynamics.csv (OutputFile.cpp:867-938), from the last WSS
+ * evaluation (Simulation.cpp:880-905). */
+int pvlbm_get_wss(pvlbm_handle h, double* out6);
+
+/* Raw WSS state per next-to-wall node: tau[3][nw] of the last evaluation and the accumulators
+ * acc[4][nw] = (sum tau_x, sum tau_y, sum tau_z, sum |tau|). Either pointer may be NULL. */
+int pvlbm_get_wss_raw(pvlbm_handle h, double* tau3, double* acc4);
+
+/* FP64 global sums over the current iteration's macroscopic state (no reference counterpart):
+ * out[0] = sum rho over all nodes, out[1] = sum rho*u_x over inlet nodes, out[2] = sum rho*u_x over
+ * outlet nodes, out[3] = number of nodes whose rho or u is not finite. */
+int pvlbm_reduce(pvlbm_handle h, double out[4]);
+
+/* Number of pulls that met the reference's "f_new != 0.0" stream gate (Simulation.cpp:222) on a
+ * link whose stale value is not kept; nonzero means a run left the reference's semantics. */
+int pvlbm_gate_events(pvlbm_handle h, int64_t* n);
+
+/* Reference inlet waveform: piecewise polynomial of period `period`
+ * (Simulation::getInletVelocity, Simulation.cpp:548-563; pulsatile.h:12-28). */
+double pvlbm_pulsatile_velocity(double t, double period);
+
+int pvlbm_destroy(pvlbm_handle h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PVLBM_H */

@@ -1,0 +1,922 @@
+// capi.cpp — host side of the C ABI declared in include/pvlbm.h.
+//
+// Set-up is split by cost: the O(N) and O(N*Q) work (voxelisation, boundary detection,
+// next-to-wall marking, normals against every triangle) runs on the GPU; the O(boundary)
+// classification runs on the host because the reference compares plane coordinates in x87
+// long double (ComplexGeometry.cpp:1370-1408), which the GPU has no type for.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <limits>
+#include <new>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/pvlbm.h"
+#include "kernels.hpp"
+#include "lbm_device.hpp"
+
+using namespace pvlbm;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct HipError {
+    hipError_t e;
+    const char* what;
+};
+
+#define HIPCHK(x)                                          \
+    do {                                                   \
+        hipError_t e_ = (x);                               \
+        if (e_ != hipSuccess) throw HipError{e_, #x};      \
+    } while (0)
+
+struct Fail {
+    int code;
+    std::string msg;
+};
+
+[[noreturn]] void fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    throw Fail{code, buf};
+}
+
+template <class T>
+T* dalloc(size_t count) {
+    void* p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc(&p, count * sizeof(T));
+    if (e != hipSuccess) fail(PVLBM_ENOMEM, "hipMalloc of %zu bytes failed: %s", count * sizeof(T), hipGetErrorString(e));
+    return static_cast<T*>(p);
+}
+
+void dfree(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+// Reference bbox rule (ComplexGeometry.cpp:142-154): x over v1..v3, y and z over v1 and v2.
+void bbox_ref(const double* tri, int64_t ntri, double bb[6]) {
+    const double inf = std::numeric_limits<double>::infinity();
+    double lo[3] = {inf, inf, inf}, hi[3] = {-inf, -inf, -inf};
+    for (int64_t t = 0; t < ntri; t++) {
+        const double* T = tri + 12 * t;
+        lo[0] = std::min({lo[0], T[0], T[3], T[6]});
+        hi[0] = std::max({hi[0], T[0], T[3], T[6]});
+        lo[1] = std::min({lo[1], T[1], T[4]});
+        hi[1] = std::max({hi[1], T[1], T[4]});
+        lo[2] = std::min({lo[2], T[2], T[5]});
+        hi[2] = std::max({hi[2], T[2], T[5]});
+    }
+    bb[0] = lo[0]; bb[1] = hi[0]; bb[2] = lo[1]; bb[3] = hi[1]; bb[4] = lo[2]; bb[5] = hi[2];
+}
+
+// Ray (x, y) of a column after the bbox nudges of adjustLineCoordinates (ComplexGeometry.cpp:856-874).
+double nudge(double v, double lo, double hi) {
+    const double rayOffset = 0.02;
+    if (v == lo) v += rayOffset;
+    if (v == hi) v -= rayOffset;
+    return v;
+}
+
+}  // namespace
+
+// =============================================================================================
+// The simulation handle
+// =============================================================================================
+
+struct pvlbm_sim {
+    std::string err;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int q = 19, storage = PVLBM_FP64;
+    int64_t nx = 0, ny = 0, nz = 0, nxp = 0, nsp = 0, guard = 0, ns = 0;
+    double origin[3] = {0, 0, 0};
+    double h = 1, tau = 1, dt = 1, C = 0;
+    // device arrays
+    uint8_t* d_kind = nullptr;
+    int32_t* d_aux = nullptr;
+    int32_t* d_tiles = nullptr;
+    int32_t ntiles = 0;
+    char* d_pop[2] = {nullptr, nullptr};  // allocation bases
+    void* A = nullptr;                    // plane-0 element 0 of the current lattice
+    void* B = nullptr;
+    int64_t nb = 0, nw = 0;
+    uint32_t* d_bmask = nullptr;
+    uint8_t* d_bclose = nullptr;
+    double* d_bprof = nullptr;
+    double* d_brho = nullptr;
+    void* d_P = nullptr;
+    void* d_M = nullptr;
+    double* d_nrm = nullptr;
+    double* d_wss = nullptr;
+    double* d_tau3 = nullptr;
+    double* d_acc = nullptr;
+    double* d_macro = nullptr;
+    double* d_partials = nullptr;
+    double* d_red = nullptr;
+    unsigned long long* d_gate = nullptr;
+    // host-side lists (unpadded box indices) and slots
+    std::vector<int64_t> lists[8];
+    std::vector<int32_t> inlet_slot;
+    int64_t iter = 0;
+    int wss_every = 1;
+    bool red_valid = false;
+
+    size_t store_bytes() const { return storage == PVLBM_FP64 ? 8 : 2; }
+    size_t real_bytes() const { return storage == PVLBM_FP64 ? 8 : 4; }
+    int nblocks() const { return (ntiles + 3) / 4; }
+    int64_t box_of_pad(int64_t n) const {
+        int64_t i = n % nxp, r = n / nxp;
+        return i + nx * r;
+    }
+    int64_t pad_of_box(int64_t b) const {
+        int64_t i = b % nx, r = b / nx;
+        return i + nxp * r;
+    }
+
+    ~pvlbm_sim() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        void* ptrs[] = {d_kind, d_aux, d_tiles, d_pop[0], d_pop[1], d_bmask, d_bclose, d_bprof, d_brho,
+                        d_P, d_M, d_nrm, d_wss, d_tau3, d_acc, d_macro, d_partials, d_red, d_gate};
+        for (void* p : ptrs) dfree(p);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    StepParams params(int flags, double scale) const {
+        StepParams p{};
+        p.A = A;
+        p.B = B;
+        p.ns = ns;
+        p.sy = nxp;
+        p.sz = nxp * ny;
+        p.kind = d_kind;
+        p.aux = d_aux;
+        p.tiles = d_tiles;
+        p.ntiles = ntiles;
+        p.flags = flags;
+        p.bmask = d_bmask;
+        p.bclose = d_bclose;
+        p.bprof = d_bprof;
+        p.brho = d_brho;
+        p.P = d_P;
+        p.pst = std::max<int64_t>(nb, 1);
+        p.M = d_M;
+        p.mst = std::max<int64_t>(nw, 1);
+        p.nrm = d_nrm;
+        p.wss = d_wss;
+        p.tau3 = d_tau3;
+        p.acc = d_acc;
+        p.omega = 1.0 / tau;
+        p.scale = scale;
+        p.C = C;
+        p.mrho = d_macro;
+        p.partials = d_partials;
+        p.gate = d_gate;
+        return p;
+    }
+
+    void launch(int flags, double scale, const double* init_rho = nullptr, const double* init_u = nullptr) {
+        StepParams p = params(flags, scale);
+        p.init_rho = init_rho;
+        p.init_u = init_u;
+        HIPCHK(launch_step(q, storage, p, nblocks(), stream));
+        std::swap(A, B);
+        if (flags & F_REDUCE) {
+            HIPCHK(launch_reduce_final(d_partials, nblocks(), d_red, stream));
+            red_valid = true;
+        }
+    }
+
+    void reset_state() {
+        HIPCHK(hipMemsetAsync(d_acc, 0, sizeof(double) * 4 * std::max<int64_t>(nw, 1), stream));
+        HIPCHK(hipMemsetAsync(d_wss, 0, sizeof(double) * std::max<int64_t>(nw, 1), stream));
+        HIPCHK(hipMemsetAsync(d_tau3, 0, sizeof(double) * 3 * std::max<int64_t>(nw, 1), stream));
+        HIPCHK(hipMemsetAsync(d_gate, 0, sizeof(unsigned long long), stream));
+        iter = 0;
+    }
+};
+
+namespace {
+
+template <class F>
+int guarded(pvlbm_sim* h, F&& fn) {
+    try {
+        fn();
+        if (h) h->err.clear();
+        return PVLBM_OK;
+    } catch (const Fail& f) {
+        (h ? h->err : g_err) = f.msg;
+        return f.code;
+    } catch (const HipError& e) {
+        (h ? h->err : g_err) = std::string(e.what) + ": " + hipGetErrorString(e.e);
+        return PVLBM_EHIP;
+    } catch (const std::bad_alloc&) {
+        (h ? h->err : g_err) = "host allocation failed";
+        return PVLBM_ENOMEM;
+    } catch (...) {
+        (h ? h->err : g_err) = "unexpected failure";
+        return PVLBM_EINVAL;
+    }
+}
+
+bool face_match(const pvlbm_face& F, const long double c[3]) {
+    // ComplexGeometry.cpp:1380-1407: |c - plane| <= tol and open windows, in long double
+    long double tolerance = F.tol;
+    long double plane = F.plane;
+    int a = (F.axis == 0) ? 1 : 0;
+    int b = (F.axis == 2) ? 1 : 2;
+    return (std::abs(c[F.axis] - plane) <= tolerance) && (F.lo[0] < c[a] && c[a] < F.hi[0]) &&
+           (F.lo[1] < c[b] && c[b] < F.hi[1]);
+}
+
+}  // namespace
+
+// =============================================================================================
+// C ABI
+// =============================================================================================
+
+extern "C" const char* pvlbm_last_error(pvlbm_handle h) { return h ? h->err.c_str() : g_err.c_str(); }
+
+extern "C" int pvlbm_device_count(int* count) {
+    return guarded(nullptr, [&] {
+        if (!count) fail(PVLBM_EINVAL, "count is NULL");
+        int c = 0;
+        hipError_t e = hipGetDeviceCount(&c);
+        *count = (e == hipSuccess) ? c : 0;
+    });
+}
+
+extern "C" int pvlbm_read_stl_ascii(const char* path, double* tri, int64_t capacity, int64_t* ntri) {
+    return guarded(nullptr, [&] {
+        if (!path || !ntri) fail(PVLBM_EINVAL, "path or ntri is NULL");
+        std::ifstream file(path);
+        if (!file.is_open()) fail(PVLBM_EIO, "cannot open file %s", path);
+        // GeometryReader.cpp:46-112: a facet is "facet normal n n n", "outer loop", three
+        // "vertex x y z", "endloop", "endfacet"; a facet whose lines do not parse is dropped.
+        std::string line, w0, w1;
+        int64_t count = 0;
+        double T[12];
+        auto next = [&](std::istringstream& ss) -> bool {
+            if (!std::getline(file, line)) line.clear();
+            ss.clear();
+            ss.str(line);
+            return true;
+        };
+        while (std::getline(file, line)) {
+            std::istringstream ss(line);
+            ss >> w0 >> w1 >> T[9] >> T[10] >> T[11];
+            if (ss.fail() || w0 != "facet" || w1 != "normal") continue;
+            next(ss);
+            ss >> w0 >> w1;
+            if (ss.fail() || w0 != "outer" || w1 != "loop") continue;
+            for (int v = 0; v < 3; v++) {
+                next(ss);
+                double x, y, z;
+                ss >> w0 >> x >> y >> z;
+                if (ss.fail() || w0 != "vertex") break;
+                T[3 * v] = x;
+                T[3 * v + 1] = y;
+                T[3 * v + 2] = z;
+            }
+            next(ss);
+            ss >> w0;
+            if (ss.fail() || w0 != "endloop") continue;
+            next(ss);
+            ss >> w0;
+            if (ss.fail() || w0 != "endfacet") continue;
+            if (tri && count < capacity) std::memcpy(tri + 12 * count, T, sizeof T);
+            count++;
+        }
+        *ntri = count;
+    });
+}
+
+extern "C" int pvlbm_bbox(const double* tri, int64_t ntri, double bbox[6]) {
+    return guarded(nullptr, [&] {
+        if (!tri || ntri <= 0 || !bbox) fail(PVLBM_EINVAL, "need at least one triangle");
+        bbox_ref(tri, ntri, bbox);
+    });
+}
+
+extern "C" int pvlbm_grid_from_bbox(const double bbox[6], double h, pvlbm_grid* grid) {
+    return guarded(nullptr, [&] {
+        if (!bbox || !grid || !(h > 0)) fail(PVLBM_EINVAL, "bad bbox/grid/h");
+        // ComplexGeometry.cpp:163-165
+        grid->origin[0] = bbox[0];
+        grid->origin[1] = bbox[2];
+        grid->origin[2] = bbox[4];
+        grid->dims[0] = (int32_t)(std::ceil((bbox[1] - bbox[0]) / h) + 1);
+        grid->dims[1] = (int32_t)(std::ceil((bbox[3] - bbox[2]) / h) + 1);
+        grid->dims[2] = (int32_t)std::ceil((bbox[5] - bbox[4]) / h);
+        grid->reserved = 0;
+    });
+}
+
+extern "C" int pvlbm_voxelize(const double* tri, int64_t ntri, double h, const pvlbm_grid* grid,
+                              pvlbm_grid* grid_out, uint8_t* fluid_out) {
+    return guarded(nullptr, [&] {
+        if (!tri || ntri <= 0 || !(h > 0) || !grid_out) fail(PVLBM_EINVAL, "bad voxelize arguments");
+        double bb[6];
+        bbox_ref(tri, ntri, bb);
+        pvlbm_grid G;
+        if (!grid) {
+            if (pvlbm_grid_from_bbox(bb, h, &G) != PVLBM_OK) fail(PVLBM_EINVAL, "%s", g_err.c_str());
+        } else {
+            G = *grid;
+            for (int a = 0; a < 3; a++) {
+                double lo = G.origin[a], hi = G.origin[a] + (double)(G.dims[a] - 1) * h;
+                bb[2 * a] = std::min(bb[2 * a], lo);
+                bb[2 * a + 1] = std::max(bb[2 * a + 1], hi);
+            }
+        }
+        if (G.dims[0] <= 0 || G.dims[1] <= 0 || G.dims[2] <= 0) fail(PVLBM_EINVAL, "empty grid");
+        *grid_out = G;
+        if (!fluid_out) return;
+        const int64_t nx = G.dims[0], ny = G.dims[1], nz = G.dims[2];
+        // candidate triangles per column: those whose xy extent, widened by a margin far above
+        // rounding, contains the column's (nudged) ray position.  The per-point test is the
+        // reference's; the margin only keeps every triangle that test could accept.
+        std::vector<double> xr(nx), yr(ny);
+        for (int64_t i = 0; i < nx; i++) xr[i] = nudge(G.origin[0] + (double)i * h, bb[0], bb[1]);
+        for (int64_t j = 0; j < ny; j++) yr[j] = nudge(G.origin[1] + (double)j * h, bb[2], bb[3]);
+        double scale = 1.0;
+        for (int a = 0; a < 6; a++) scale = std::max(scale, std::fabs(bb[a]));
+        const double margin = 1e-6 * scale;
+        const int64_t slack = 1 + (int64_t)std::ceil(0.02 / h);
+        std::vector<int64_t> count((size_t)(nx * ny + 1), 0);
+        std::vector<std::array<int64_t, 4>> ranges((size_t)ntri);
+        auto col_range = [&](int64_t t, int64_t r[4], double box[4]) {
+            const double* T = tri + 12 * t;
+            box[0] = std::min({T[0], T[3], T[6]}) - margin;
+            box[1] = std::max({T[0], T[3], T[6]}) + margin;
+            box[2] = std::min({T[1], T[4], T[7]}) - margin;
+            box[3] = std::max({T[1], T[4], T[7]}) + margin;
+            r[0] = std::max<int64_t>(0, (int64_t)std::floor((box[0] - G.origin[0]) / h) - slack);
+            r[1] = std::min<int64_t>(nx - 1, (int64_t)std::ceil((box[1] - G.origin[0]) / h) + slack);
+            r[2] = std::max<int64_t>(0, (int64_t)std::floor((box[2] - G.origin[1]) / h) - slack);
+            r[3] = std::min<int64_t>(ny - 1, (int64_t)std::ceil((box[3] - G.origin[1]) / h) + slack);
+        };
+        for (int pass = 0; pass < 2; pass++) {
+            std::vector<int32_t> tri_ids;
+            std::vector<int64_t> fill;
+            if (pass == 1) {
+                for (int64_t c = 0; c < nx * ny; c++) count[c + 1] += count[c];
+                tri_ids.resize((size_t)count[nx * ny]);
+                fill.assign(count.begin(), count.end() - 1);
+            }
+            for (int64_t t = 0; t < ntri; t++) {
+                int64_t r[4];
+                double box[4];
+                col_range(t, r, box);
+                for (int64_t j = r[2]; j <= r[3]; j++) {
+                    if (yr[j] < box[2] || yr[j] > box[3]) continue;
+                    for (int64_t i = r[0]; i <= r[1]; i++) {
+                        if (xr[i] < box[0] || xr[i] > box[1]) continue;
+                        if (pass == 0) count[i + nx * j + 1]++;
+                        else tri_ids[(size_t)fill[i + nx * j]++] = (int32_t)t;
+                    }
+                }
+            }
+            if (pass == 1) {
+                hipStream_t s;
+                HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+                double* d_tri = nullptr;
+                int64_t* d_off = nullptr;
+                int32_t* d_ids = nullptr;
+                uint8_t* d_fluid = nullptr;
+                try {
+                    d_tri = dalloc<double>((size_t)(12 * ntri));
+                    d_off = dalloc<int64_t>(count.size());
+                    d_ids = dalloc<int32_t>(tri_ids.size());
+                    d_fluid = dalloc<uint8_t>((size_t)(nx * ny * nz));
+                    HIPCHK(hipMemcpyAsync(d_tri, tri, sizeof(double) * 12 * ntri, hipMemcpyHostToDevice, s));
+                    HIPCHK(hipMemcpyAsync(d_off, count.data(), sizeof(int64_t) * count.size(), hipMemcpyHostToDevice, s));
+                    if (!tri_ids.empty())
+                        HIPCHK(hipMemcpyAsync(d_ids, tri_ids.data(), sizeof(int32_t) * tri_ids.size(), hipMemcpyHostToDevice, s));
+                    VoxParams p{};
+                    p.tri = d_tri;
+                    p.col_off = d_off;
+                    p.col_tri = d_ids;
+                    p.ox = G.origin[0];
+                    p.oy = G.origin[1];
+                    p.oz = G.origin[2];
+                    p.h = h;
+                    for (int a = 0; a < 6; a++) p.bb[a] = bb[a];
+                    p.nx = (int32_t)nx;
+                    p.ny = (int32_t)ny;
+                    p.nz = (int32_t)nz;
+                    p.fluid = d_fluid;
+                    HIPCHK(launch_voxelize(p, s));
+                    HIPCHK(hipMemcpyAsync(fluid_out, d_fluid, (size_t)(nx * ny * nz), hipMemcpyDeviceToHost, s));
+                    HIPCHK(hipStreamSynchronize(s));
+                } catch (...) {
+                    (void)hipStreamSynchronize(s);
+                    dfree(d_tri); dfree(d_off); dfree(d_ids); dfree(d_fluid);
+                    (void)hipStreamDestroy(s);
+                    throw;
+                }
+                dfree(d_tri); dfree(d_off); dfree(d_ids); dfree(d_fluid);
+                (void)hipStreamDestroy(s);
+            }
+        }
+    });
+}
+
+extern "C" int pvlbm_create(const uint8_t* fluid, const pvlbm_grid* grid, double h, int lattice,
+                            int storage, const pvlbm_face* faces, int nfaces, const double* tri,
+                            int64_t ntri, double tau, double dt, pvlbm_handle* out) {
+    pvlbm_sim* S = nullptr;
+    int rc = guarded(nullptr, [&] {
+        if (!fluid || !grid || !out) fail(PVLBM_EINVAL, "fluid, grid and out are required");
+        if (lattice != 19 && lattice != 27) fail(PVLBM_EINVAL, "lattice must be 19 or 27, got %d", lattice);
+        if (storage < PVLBM_FP64 || storage > PVLBM_BF16S) fail(PVLBM_EINVAL, "bad storage %d", storage);
+        if (!(h > 0) || !(tau > 0.5) || !(dt > 0)) fail(PVLBM_EINVAL, "need h > 0, tau > 0.5, dt > 0");
+        if (nfaces < 0 || (nfaces > 0 && !faces)) fail(PVLBM_EINVAL, "bad faces");
+        if (ntri < 0 || (ntri > 0 && !tri)) fail(PVLBM_EINVAL, "bad triangles");
+        for (int f = 0; f < nfaces; f++) {
+            const pvlbm_face& F = faces[f];
+            if (F.role < PVLBM_INLET || F.role > PVLBM_OUTLET3) fail(PVLBM_EINVAL, "face %d: bad role", f);
+            if (F.axis < 0 || F.axis > 2) fail(PVLBM_EINVAL, "face %d: bad axis", f);
+            bool inlet = F.role == PVLBM_INLET;
+            if (inlet != (F.closure == PVLBM_INLET_XMIN)) fail(PVLBM_EINVAL, "face %d: inlets take INLET_XMIN, outlets a pressure closure", f);
+            if (F.closure < PVLBM_INLET_XMIN || F.closure > PVLBM_OUTLET_XMAX) fail(PVLBM_EINVAL, "face %d: bad closure", f);
+            if (lattice == 27 && (F.closure == PVLBM_OUTLET_YMAX || F.closure == PVLBM_OUTLET_ZMIN))
+                fail(PVLBM_EINVAL, "face %d: the y-max/z-min outlet closures exist for D3Q19 only", f);
+            if (!inlet && !(F.rho > 0)) fail(PVLBM_EINVAL, "face %d: outlet density must be > 0", f);
+        }
+        const int64_t nx = grid->dims[0], ny = grid->dims[1], nz = grid->dims[2];
+        if (nx <= 0 || ny <= 0 || nz <= 0) fail(PVLBM_EINVAL, "empty grid");
+        const int64_t nxp = (nx + 63) / 64 * 64;
+        const int64_t nsp = nxp * ny * nz;
+        if (nsp / 64 > (int64_t)INT32_MAX) fail(PVLBM_EINVAL, "grid too large");
+
+        S = new pvlbm_sim();
+        S->q = lattice;
+        S->storage = storage;
+        S->nx = nx; S->ny = ny; S->nz = nz; S->nxp = nxp; S->nsp = nsp;
+        S->guard = nxp * ny + nxp + 64;
+        S->ns = nsp + 2 * S->guard;
+        for (int a = 0; a < 3; a++) S->origin[a] = grid->origin[a];
+        S->h = h; S->tau = tau; S->dt = dt;
+        {
+            // Simulation.cpp:880-881
+            double phys_den = 1060;
+            double dtdt = dt * dt;
+            double dxdx = h * h * 1e-6;
+            S->C = phys_den * dtdt / dxdx;
+        }
+        HIPCHK(hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking));
+        HIPCHK(hipEventCreate(&S->ev0));
+        HIPCHK(hipEventCreate(&S->ev1));
+        hipStream_t st = S->stream;
+        const int q = lattice;
+        const int (*cx) = (q == 19) ? Lat<19>::cx : Lat<27>::cx;
+        const int (*cy) = (q == 19) ? Lat<19>::cy : Lat<27>::cy;
+        const int (*cz) = (q == 19) ? Lat<19>::cz : Lat<27>::cz;
+
+        // padded fluid flags -> boundary flags on the GPU
+        std::vector<uint8_t> fpad((size_t)nsp, 0);
+        for (int64_t r = 0; r < ny * nz; r++) std::memcpy(&fpad[r * nxp], fluid + r * nx, (size_t)nx);
+        uint8_t* d_fluid = dalloc<uint8_t>((size_t)nsp);
+        uint8_t* d_bflag = nullptr;
+        std::vector<uint8_t> bflag((size_t)nsp);
+        try {
+            d_bflag = dalloc<uint8_t>((size_t)nsp);
+            HIPCHK(hipMemcpyAsync(d_fluid, fpad.data(), (size_t)nsp, hipMemcpyHostToDevice, st));
+            HIPCHK(launch_boundary(q, d_fluid, nxp, ny, nz, nx, d_bflag, st));
+            HIPCHK(hipMemcpyAsync(bflag.data(), d_bflag, (size_t)nsp, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+        } catch (...) {
+            dfree(d_fluid); dfree(d_bflag);
+            throw;
+        }
+        dfree(d_fluid);
+        dfree(d_bflag);
+
+        // classification of boundary nodes (host, long double) and the boundary table
+        std::vector<uint8_t> kind((size_t)nsp, K_SOLID);
+        std::vector<int32_t> aux((size_t)nsp, -1);
+        std::vector<uint32_t> bmask;
+        std::vector<uint8_t> bclose;
+        std::vector<double> bprof, brho;
+        std::vector<int64_t> bnode;
+        std::vector<int> brole;
+        for (int64_t n = 0; n < nsp; n++) {
+            if (!fpad[n]) continue;
+            kind[n] = K_FLUID;
+            if (!bflag[n]) continue;
+            const int64_t i = n % nxp, j = (n / nxp) % ny, k = n / (nxp * ny);
+            long double c3[3] = {(long double)(grid->origin[0] + (double)i * h),
+                                 (long double)(grid->origin[1] + (double)j * h),
+                                 (long double)(grid->origin[2] + (double)k * h)};
+            int matched = -1;
+            for (int role = PVLBM_INLET; role <= PVLBM_OUTLET3 && matched < 0; role++)
+                for (int f = 0; f < nfaces; f++)
+                    if (faces[f].role == role && face_match(faces[f], c3)) {
+                        matched = f;
+                        break;
+                    }
+            uint32_t m = 0;
+            for (int d = 0; d < q; d++) {
+                int64_t a = i + cx[d], b = j + cy[d], c = k + cz[d];
+                bool inside = a >= 0 && a < nx && b >= 0 && b < ny && c >= 0 && c < nz;
+                if (!inside || !fpad[a + nxp * (b + ny * c)]) m |= 1u << d;
+            }
+            aux[n] = (int32_t)bnode.size();
+            bnode.push_back(n);
+            bmask.push_back(m);
+            if (matched < 0) {
+                kind[n] = K_WALL;
+                bclose.push_back(BC_WALL);
+                brole.push_back(-1);
+                bprof.push_back(0.0);
+                brho.push_back(1.0);
+            } else {
+                const pvlbm_face& F = faces[matched];
+                kind[n] = (F.role == PVLBM_INLET) ? K_INLET : K_OUTLET;
+                bclose.push_back((uint8_t)(F.closure + 1));
+                brole.push_back(F.role);
+                bprof.push_back(1.0);
+                brho.push_back(F.rho);
+            }
+        }
+        S->nb = (int64_t)bnode.size();
+
+        // next-to-wall marking on the GPU
+        S->d_kind = dalloc<uint8_t>((size_t)nsp);
+        HIPCHK(hipMemcpyAsync(S->d_kind, kind.data(), (size_t)nsp, hipMemcpyHostToDevice, st));
+        HIPCHK(launch_ntw(q, S->d_kind, nxp, ny, nz, st));
+        HIPCHK(hipMemcpyAsync(kind.data(), S->d_kind, (size_t)nsp, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+
+        // lists, next-to-wall slots, active tiles
+        std::vector<int32_t> tiles;
+        std::vector<double> xyz;
+        std::vector<int64_t> wnode;
+        for (int64_t n = 0; n < nsp; n++) {
+            const uint8_t kd = kind[n];
+            if (kd == K_SOLID) continue;
+            const int64_t b = S->box_of_pad(n);
+            S->lists[PVLBM_LIST_FLUID].push_back(b);
+            if (tiles.empty() || tiles.back() != (int32_t)(n / 64)) tiles.push_back((int32_t)(n / 64));
+            if (kd == K_NTW) {
+                aux[n] = (int32_t)wnode.size();
+                wnode.push_back(n);
+                S->lists[PVLBM_LIST_NTW].push_back(b);
+            } else if (kd >= K_INLET) {
+                const int32_t slot = aux[n];
+                if (kd == K_WALL) S->lists[PVLBM_LIST_WALL].push_back(b);
+                else if (kd == K_INLET) {
+                    S->lists[PVLBM_LIST_INLET].push_back(b);
+                    S->inlet_slot.push_back(slot);
+                } else {
+                    S->lists[PVLBM_LIST_OUTLET1 + brole[slot] - 1].push_back(b);
+                }
+            }
+        }
+        S->nw = (int64_t)wnode.size();
+        S->ntiles = (int32_t)tiles.size();
+        for (int32_t t : tiles) S->lists[PVLBM_LIST_TILES].push_back(t);
+        xyz.resize((size_t)(3 * S->nw));
+        for (int64_t w = 0; w < S->nw; w++) {
+            const int64_t n = wnode[w];
+            const int64_t i = n % nxp, j = (n / nxp) % ny, k = n / (nxp * ny);
+            xyz[w] = grid->origin[0] + (double)i * h;
+            xyz[S->nw + w] = grid->origin[1] + (double)j * h;
+            xyz[2 * S->nw + w] = grid->origin[2] + (double)k * h;
+        }
+
+        // device tables
+        const int64_t nb1 = std::max<int64_t>(S->nb, 1), nw1 = std::max<int64_t>(S->nw, 1);
+        S->d_aux = dalloc<int32_t>((size_t)nsp);
+        S->d_tiles = dalloc<int32_t>(tiles.size());
+        S->d_bmask = dalloc<uint32_t>((size_t)nb1);
+        S->d_bclose = dalloc<uint8_t>((size_t)nb1);
+        S->d_bprof = dalloc<double>((size_t)nb1);
+        S->d_brho = dalloc<double>((size_t)nb1);
+        S->d_P = dalloc<char>((size_t)(q * nb1) * S->real_bytes());
+        S->d_M = dalloc<char>((size_t)(4 * nw1) * S->real_bytes());
+        S->d_nrm = dalloc<double>((size_t)(3 * nw1));
+        S->d_wss = dalloc<double>((size_t)nw1);
+        S->d_tau3 = dalloc<double>((size_t)(3 * nw1));
+        S->d_acc = dalloc<double>((size_t)(4 * nw1));
+        S->d_macro = dalloc<double>((size_t)(4 * S->ns));
+        S->d_partials = dalloc<double>((size_t)(4 * std::max(S->nblocks(), 1)));
+        S->d_red = dalloc<double>(4);
+        S->d_gate = dalloc<unsigned long long>(1);
+        const size_t pop_bytes = (size_t)q * (size_t)S->ns * S->store_bytes();
+        for (int c = 0; c < 2; c++) {
+            S->d_pop[c] = dalloc<char>(pop_bytes);
+            HIPCHK(hipMemsetAsync(S->d_pop[c], 0, pop_bytes, st));
+        }
+        S->A = S->d_pop[0] + S->guard * S->store_bytes();
+        S->B = S->d_pop[1] + S->guard * S->store_bytes();
+        HIPCHK(hipMemcpyAsync(S->d_aux, aux.data(), sizeof(int32_t) * nsp, hipMemcpyHostToDevice, st));
+        if (!tiles.empty())
+            HIPCHK(hipMemcpyAsync(S->d_tiles, tiles.data(), sizeof(int32_t) * tiles.size(), hipMemcpyHostToDevice, st));
+        if (S->nb) {
+            HIPCHK(hipMemcpyAsync(S->d_bmask, bmask.data(), sizeof(uint32_t) * S->nb, hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(S->d_bclose, bclose.data(), S->nb, hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(S->d_bprof, bprof.data(), sizeof(double) * S->nb, hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(S->d_brho, brho.data(), sizeof(double) * S->nb, hipMemcpyHostToDevice, st));
+        }
+        HIPCHK(hipMemsetAsync(S->d_nrm, 0, sizeof(double) * 3 * nw1, st));
+        HIPCHK(hipMemsetAsync(S->d_M, 0, (size_t)(4 * nw1) * S->real_bytes(), st));
+        HIPCHK(hipMemsetAsync(S->d_P, 0, (size_t)(q * nb1) * S->real_bytes(), st));
+        HIPCHK(hipMemsetAsync(S->d_macro, 0, sizeof(double) * 4 * S->ns, st));
+        // normals of the next-to-wall nodes against every triangle
+        if (S->nw > 0 && ntri > 0) {
+            double *d_tri = nullptr, *d_prep = nullptr, *d_xyz = nullptr;
+            try {
+                d_tri = dalloc<double>((size_t)(12 * ntri));
+                d_prep = dalloc<double>((size_t)(7 * ntri));
+                d_xyz = dalloc<double>((size_t)(3 * S->nw));
+                HIPCHK(hipMemcpyAsync(d_tri, tri, sizeof(double) * 12 * ntri, hipMemcpyHostToDevice, st));
+                HIPCHK(hipMemcpyAsync(d_xyz, xyz.data(), sizeof(double) * 3 * S->nw, hipMemcpyHostToDevice, st));
+                HIPCHK(launch_tri_prep(d_tri, ntri, d_prep, st));
+                HIPCHK(launch_normals(d_prep, ntri, d_xyz, S->nw, S->d_nrm, st));
+                HIPCHK(hipStreamSynchronize(st));
+            } catch (...) {
+                (void)hipStreamSynchronize(st);
+                dfree(d_tri); dfree(d_prep); dfree(d_xyz);
+                throw;
+            }
+            dfree(d_tri); dfree(d_prep); dfree(d_xyz);
+        }
+        S->reset_state();
+        // f = w everywhere, first collision (Subdomain.cpp:549-564, Simulation.cpp:43-208)
+        S->launch(F_INIT | F_MACRO | F_REDUCE, 1.0);
+        HIPCHK(hipStreamSynchronize(st));
+        *out = S;
+    });
+    if (rc != PVLBM_OK) {
+        delete S;
+        if (out) *out = nullptr;
+    }
+    return rc;
+}
+
+extern "C" int pvlbm_set_tau(pvlbm_handle h, double tau) {
+    if (!h) return PVLBM_EINVAL;
+    return guarded(h, [&] {
+        if (!(tau > 0.5)) fail(PVLBM_EINVAL, "tau must exceed 0.5");
+        h->tau = tau;
+    });
+}
+
+extern "C" int pvlbm_set_dt(pvlbm_handle h, double dt) {
+    if (!h) return PVLBM_EINVAL;
+    return guarded(h, [&] {
+        if (!(dt > 0)) fail(PVLBM_EINVAL, "dt must be positive");
+        h->dt = dt;
+        // Simulation.cpp:880-881
+        double phys_den = 1060;
+        double dtdt = dt * dt;
+        double dxdx = h->h * h->h * 1e-6;
+        h->C = phys_den * dtdt / dxdx;
+    });
+}
+
+extern "C" int pvlbm_count(pvlbm_handle h, int list, int64_t* n) {
+    if (!h) return PVLBM_EINVAL;
+    return guarded(h, [&] {
+        if (list < 0 || list > PVLBM_LIST_TILES || !n) fail(PVLBM_EINVAL, "bad list id");
+        *n = (int64_t)h->lists[list].size();
+    });
+}
+
+extern "C" int pvlbm_list(pvlbm_handle h, int list, int64_t* idx) {
+    if (!h) return PVLBM_EINVAL;
+    return guarded(h, [&] {
+        if (list < 0 || list > PVLBM_LIST_TILES || !idx) fail(PVLBM_EINVAL, "bad list id");
+        std::copy(h->lists[list].begin(), h->lists[list].end(), idx);
+    });
+}
+
+extern "C" int pvlbm_get_kinds(pvlbm_handle h, uint8_t* kinds) {
+    if (!h) return PVLBM_EINVAL;
+    return guarded(h, [&] {
+        if (!kinds) fail(PVLBM_EINVAL, "kinds is NULL");
+        std::vector<uint8_t> k((size_t)h->nsp);
+        HIPCHK(hipMemcpyAsync(k.data(), h->d_kind, (size_t)h->nsp, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        for (int64_t r = 0; r < h->ny * h->nz; r++) std::memcpy(kinds + r * h->nx, &k[r * h->nxp], (size_t)h->nx);
+    });
+}
+
+extern "C" int pvlbm_set_inlet_profile(pvlbm_handle h, const double* u) {
+    if (!h) return PVLBM_EINVAL;
+    return guarded(h, [&] {
+        if (!u) fail(PVLBM_EINVAL, "u is NULL");
+        std::vector<double> prof((size_t)std::max<int64_t>(h->nb, 1), 0.0);
+        HIPCHK(hipMemcpyAsync(prof.data(), h->d_bprof, sizeof(double) * h->nb, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        for (size_t i = 0; i < h->inlet_slot.size(); i++) prof[h->inlet_slot[i]] = u[i];
+        HIPCHK(hipMemcpyAsync(h->d_bprof, prof.data(), sizeof(double) * h->nb, hipMemcpyHostToDevice, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+    });
+}
+
+extern "C" int pvlbm_init_fields(pvlbm_handle h, const double* rho, const double* u) {
+    if (!h) return PVLBM_EINVAL;
+    return guarded(h, [&] {
+        if (rho && !u) fail(PVLBM_EINVAL, "u is required with rho");
+        double* d_in = nullptr;
+        try {
+            if (rho) {
+                // padded [4][ns] staging: rho plane then three velocity planes
+                std::vector<double> pad((size_t)(4 * h->ns), 0.0);
+                for (int64_t r = 0; r < h->ny * h->nz; r++) {
+                    std::memcpy(&pad[r * h->nxp], rho + r * h->nx, sizeof(double) * h->nx);
+                    for (int c = 0; c < 3; c++)
+                        std::memcpy(&pad[(c + 1) * h->ns + r * h->nxp], u + c * h->nx * h->ny * h->nz + r * h->nx,
+                                    sizeof(double) * h->nx);
+                }
+                d_in = dalloc<double>(pad.size());
+                HIPCHK(hipMemcpyAsync(d_in, pad.data(), sizeof(double) * pad.size(), hipMemcpyHostToDevice, h->stream));
+            }
+            h->reset_state();
+            h->launch(F_INIT | F_MACRO | F_REDUCE, 1.0, d_in, d_in ? d_in + h->ns : nullptr);
+            HIPCHK(hipStreamSynchronize(h->stream));
+        } catch (...) {
+            (void)hipStreamSynchronize(h->stream);
+            dfree(d_in);
+            throw;
+        }
+        dfree(d_in);
+    });
+}
+
+extern "C" int pvlbm_set_wss_every(pvlbm_handle h, int every) {
+    if (!h) return PVLBM_EINVAL;
+    return guarded(h, [&] {
+        if (every < 1) fail(PVLBM_EINVAL, "every must be >= 1");
+        h->wss_every = every;
+    });
+}
+
+namespace {
+void run_steps(pvlbm_sim* h, int n, const double* scale) {
+    for (int s = 0; s < n; s++) {
+        int flags = (h->iter % h->wss_every == 0) ? F_WSS : 0;
+        if (s == n - 1) flags |= F_MACRO | F_REDUCE;
+        h->launch(flags, scale ? scale[s] : 1.0);
+        h->iter++;
+    }
+}
+}  // namespace
+
+extern "C" int pvlbm_step(pvlbm_handle h, int n, const double* inlet_scale) {
+    if (!h) return PVLBM_EINVAL;
+    return guarded(h, [&] {
+        if (n < 0) fail(PVLBM_EINVAL, "n < 0");
+        run_steps(h, n, inlet_scale);
+        HIPCHK(hipStreamSynchronize(h->stream));
+    });
+}
+
+extern "C" int pvlbm_step_timed(pvlbm_handle h, int n, const double* inlet_scale, double* ms) {
+    if (!h) return PVLBM_EINVAL;
+    return guarded(h, [&] {
+        if (n < 0 || !ms) fail(PVLBM_EINVAL, "bad arguments");
+        HIPCHK(hipEventRecord(h->ev0, h->stream));
+        run_steps(h, n, inlet_scale);
+        HIPCHK(hipEventRecord(h->ev1, h->stream));
+        HIPCHK(hipEventSynchronize(h->ev1));
+        float f = 0;
+        HIPCHK(hipEventElapsedTime(&f, h->ev0, h->ev1));
+        *ms = f;
+    });
+}
+
+extern "C" int pvlbm_iteration(pvlbm_handle h, int64_t* it) {
+    if (!h || !it) return PVLBM_EINVAL;
+    *it = h->iter;
+    return PVLBM_OK;
+}
+
+extern "C" int pvlbm_get_macros(pvlbm_handle h, double* rho, double* u) {
+    if (!h) return PVLBM_EINVAL;
+    return guarded(h, [&] {
+        std::vector<double> m((size_t)(4 * h->ns));
+        HIPCHK(hipMemcpyAsync(m.data(), h->d_macro, sizeof(double) * m.size(), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        std::vector<uint8_t> k((size_t)h->nsp);
+        HIPCHK(hipMemcpy(k.data(), h->d_kind, (size_t)h->nsp, hipMemcpyDeviceToHost));
+        const int64_t nbox = h->nx * h->ny * h->nz;
+        for (int64_t r = 0; r < h->ny * h->nz; r++)
+            for (int64_t i = 0; i < h->nx; i++) {
+                const int64_t n = r * h->nxp + i, b = r * h->nx + i;
+                const bool node = k[n] != K_SOLID;
+                if (rho) rho[b] = node ? m[n] : 0.0;
+                if (u)
+                    for (int c = 0; c < 3; c++) u[c * nbox + b] = node ? m[(c + 1) * h->ns + n] : 0.0;
+            }
+    });
+}
+
+extern "C" int pvlbm_get_wss(pvlbm_handle h, double* out6) {
+    if (!h) return PVLBM_EINVAL;
+    return guarded(h, [&] {
+        if (!out6) fail(PVLBM_EINVAL, "out6 is NULL");
+        const int64_t nw = h->nw, nw1 = std::max<int64_t>(nw, 1);
+        std::vector<double> wss((size_t)nw1), acc((size_t)(4 * nw1));
+        HIPCHK(hipMemcpyAsync(wss.data(), h->d_wss, sizeof(double) * nw1, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipMemcpyAsync(acc.data(), h->d_acc, sizeof(double) * 4 * nw1, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        for (int64_t w = 0; w < nw; w++) {
+            const int64_t b = h->lists[PVLBM_LIST_NTW][w];
+            const int64_t i = b % h->nx, j = (b / h->nx) % h->ny, k = b / (h->nx * h->ny);
+            const double a0 = acc[w], a1 = acc[nw1 + w], a2 = acc[2 * nw1 + w], am = acc[3 * nw1 + w];
+            // ComputeOSIAndTAWSS (Simulation.cpp:899-905), period 0.9 (pulsatile.h:12)
+            const double tawss = am / 0.9;
+            const double norm_acc = std::sqrt(a0 * a0 + a1 * a1 + a2 * a2);
+            const double osi = 0.5 * (1.0 - (norm_acc / am));
+            double* o = out6 + 6 * w;
+            o[0] = h->origin[0] + (double)i * h->h;
+            o[1] = h->origin[1] + (double)j * h->h;
+            o[2] = h->origin[2] + (double)k * h->h;
+            o[3] = wss[w];
+            o[4] = osi;
+            o[5] = tawss;
+        }
+    });
+}
+
+extern "C" int pvlbm_get_wss_raw(pvlbm_handle h, double* tau3, double* acc4) {
+    if (!h) return PVLBM_EINVAL;
+    return guarded(h, [&] {
+        const int64_t nw = h->nw, nw1 = std::max<int64_t>(nw, 1);
+        std::vector<double> t((size_t)(3 * nw1)), a((size_t)(4 * nw1));
+        HIPCHK(hipMemcpyAsync(t.data(), h->d_tau3, sizeof(double) * t.size(), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipMemcpyAsync(a.data(), h->d_acc, sizeof(double) * a.size(), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        for (int64_t w = 0; w < nw; w++) {
+            if (tau3)
+                for (int c = 0; c < 3; c++) tau3[c * nw + w] = t[c * nw1 + w];
+            if (acc4)
+                for (int c = 0; c < 4; c++) acc4[c * nw + w] = a[c * nw1 + w];
+        }
+    });
+}
+
+extern "C" int pvlbm_reduce(pvlbm_handle h, double out[4]) {
+    if (!h) return PVLBM_EINVAL;
+    return guarded(h, [&] {
+        if (!out) fail(PVLBM_EINVAL, "out is NULL");
+        if (!h->red_valid) fail(PVLBM_ESTATE, "no reduction computed yet");
+        HIPCHK(hipMemcpyAsync(out, h->d_red, sizeof(double) * 4, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+    });
+}
+
+extern "C" int pvlbm_gate_events(pvlbm_handle h, int64_t* n) {
+    if (!h) return PVLBM_EINVAL;
+    return guarded(h, [&] {
+        if (!n) fail(PVLBM_EINVAL, "n is NULL");
+        unsigned long long v = 0;
+        HIPCHK(hipMemcpyAsync(&v, h->d_gate, sizeof v, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        *n = (int64_t)v;
+    });
+}
+
+extern "C" double pvlbm_pulsatile_velocity(double x, double period) {
+    // Simulation::getInletVelocity (Simulation.cpp:548-563) with the Pulsatile polynomials
+    // (pulsatile.h:16-28); x^n through pow as the reference writes it.
+    double vel = 0.0;
+    double t = std::fmod(x, period);
+    if (t >= 0 && t < (0.1 * period)) {
+        vel = (-8.579e+08 * std::pow(t, 8) + 4.059e+08 * std::pow(t, 7) - 7.817e+07 * std::pow(t, 6) +
+               7.994e+06 * std::pow(t, 5) - 4.835e+05 * std::pow(t, 4) + 1.782e+04 * std::pow(t, 3) -
+               321.6 * std::pow(t, 2) + 3.215 * t + 0.1423);
+    } else if (t >= (0.1 * period) && t < (0.3 * period)) {
+        vel = (-4.32e+06 * std::pow(t, 8) + 7.94e+06 * std::pow(t, 7) - 6.298e+06 * std::pow(t, 6) +
+               2.82e+06 * std::pow(t, 5) - 7.809e+05 * std::pow(t, 4) + 1.37e+05 * std::pow(t, 3) -
+               1.484e+04 * std::pow(t, 2) + 901.9 * t - 22.87);
+    } else if (t >= (0.3 * period) && t < (0.9 * period)) {
+        vel = (-5.123 * std::pow(t, 4) + 14.15 * std::pow(t, 3) - 14.21 * std::pow(t, 2) + 5.865 * t - 0.5696);
+    }
+    return vel;
+}
+
+extern "C" int pvlbm_destroy(pvlbm_handle h) {
+    delete h;
+    return PVLBM_OK;
+}

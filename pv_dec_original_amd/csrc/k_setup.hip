@@ -1,0 +1,243 @@
+// k_setup.hip — set-up kernels for gfx950: ray-cast voxelisation, boundary detection,
+// next-to-wall marking and boundary normals.
+
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+#include "lbm_device.hpp"
+
+namespace pvlbm {
+
+namespace {
+
+// Geometry::isPointInsideTriangle (ComplexGeometry.cpp:620-674), same operation order.
+__device__ __forceinline__ bool point_in_triangle(double x, double y, double z, const double* T) {
+    double x1 = T[0], y1 = T[1], z1 = T[2];
+    double x2 = T[3], y2 = T[4], z2 = T[5];
+    double x3 = T[6], y3 = T[7], z3 = T[8];
+    double dirX = 0.0, dirY = 0.0, dirZ = 1.0;
+    double uX = x2 - x1, uY = y2 - y1, uZ = z2 - z1;
+    double vX = x3 - x1, vY = y3 - y1, vZ = z3 - z1;
+    double nX = uY * vZ - uZ * vY;
+    double nY = uZ * vX - uX * vZ;
+    double nZ = uX * vY - uY * vX;
+    double nLengthSq = nX * nX + nY * nY + nZ * nZ;
+    const double tol = 1e-16;
+    if (nLengthSq < tol * tol) return false;
+    double w0X = x - x1, w0Y = y - y1, w0Z = z - z1;
+    double a = -(nX * w0X + nY * w0Y + nZ * w0Z);
+    double b = nX * dirX + nY * dirY + nZ * dirZ;
+    if (fabs(b) < tol) return false;
+    double pos = a / b;
+    double ix = x + pos * dirX;
+    double iy = y + pos * dirY;
+    double iz = z + pos * dirZ;
+    double uu = uX * uX + uY * uY + uZ * uZ;
+    double uv = uX * vX + uY * vY + uZ * vZ;
+    double vv = vX * vX + vY * vY + vZ * vZ;
+    double wX = ix - x1, wY = iy - y1, wZ = iz - z1;
+    double wu = wX * uX + wY * uY + wZ * uZ;
+    double wv = wX * vX + wY * vY + wZ * vZ;
+    double D = uv * uv - uu * vv;
+    double s = (uv * wv - vv * wu) / D;
+    if (s < 0.0 || s > 1.0) return false;
+    double t = (uv * wu - uu * wv) / D;
+    if (t < 0.0 || (s + t) > 1.0) return false;
+    return true;
+}
+
+// One triangle of Geometry::countIntersectinsTriangle (ComplexGeometry.cpp:807-845).
+__device__ __forceinline__ bool crosses(const double* T, const double s[3], const double e[3]) {
+    const double threshold = 1e-16;
+    double px = s[0], py = s[1], pz = s[2];
+    double dx = e[0] - s[0], dy = e[1] - s[1], dz = e[2] - s[2];
+    double nx = T[9], ny = T[10], nz = T[11];
+    double qx = T[0], qy = T[1], qz = T[2];
+    double numerator = (qx - px) * nx + (qy - py) * ny + (qz - pz) * nz;
+    double denominator = dx * nx + dy * ny + dz * nz;
+    if (fabs(denominator) < threshold) return false;
+    double t = numerator / denominator;
+    if (t >= 0.0 && t <= 1.0) {
+        double ix = px + t * dx;
+        double iy = py + t * dy;
+        double iz = pz + t * dz;
+        if (point_in_triangle(ix, iy, iz, T)) return true;
+    }
+    return false;
+}
+
+// One wave per (i, j) column; its lanes walk the column's points in z.  The candidate triangles
+// of a column are the same for every lane, so their loads are wave-uniform.
+__global__ __launch_bounds__(64) void k_voxelize(const VoxParams p) {
+    const int64_t col = blockIdx.x;
+    const int64_t i = col % p.nx, j = col / p.nx;
+    const int64_t c0 = p.col_off[col], c1 = p.col_off[col + 1];
+    const double rayOffset = 0.02;
+    for (int64_t k = threadIdx.x; k < p.nz; k += 64) {
+        // point = min + idx*spacing (ComplexGeometry.cpp:337-340); ray of labelPoints (:989-1003)
+        // and adjustLineCoordinates case 0 (:855-884)
+        double x = p.ox + (double)i * p.h;
+        double y = p.oy + (double)j * p.h;
+        double z = p.oz + (double)k * p.h;
+        double s[3] = {x, y, z + p.h / 2};
+        double rayLength = p.bb[5] - p.bb[4];
+        double e[3] = {x, y, z + rayLength + 0.1};
+        if (s[0] == p.bb[0]) { s[0] += rayOffset; e[0] += rayOffset; }
+        if (s[0] == p.bb[1]) { s[0] -= rayOffset; e[0] -= rayOffset; }
+        if (s[1] == p.bb[2]) { s[1] += rayOffset; e[1] += rayOffset; }
+        if (s[1] == p.bb[3]) { s[1] -= rayOffset; e[1] -= rayOffset; }
+        if (s[2] == p.bb[4]) { s[2] += rayOffset; }
+        if (e[2] == p.bb[5]) { e[2] -= rayOffset; }
+        int count = 0;
+        for (int64_t c = c0; c < c1; c++) {
+            const double* T = p.tri + 12 * (int64_t)p.col_tri[c];
+            if (crosses(T, s, e)) count++;
+        }
+        p.fluid[i + (int64_t)p.nx * (j + (int64_t)p.ny * k)] = (count % 2 == 1) ? 1 : 0;
+    }
+}
+
+// findBoundaryPoints (ComplexGeometry.cpp:1259-1300) over the Q directions: a fluid node is a
+// boundary node when a Q neighbour is outside the box or not fluid.
+template <int Q>
+__global__ __launch_bounds__(256) void k_boundary(const uint8_t* fluid, int64_t nxp, int64_t ny,
+                                                 int64_t nz, int64_t nx, uint8_t* bflag) {
+    const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t total = nxp * ny * nz;
+    if (n >= total) return;
+    const int64_t i = n % nxp, j = (n / nxp) % ny, k = n / (nxp * ny);
+    if (!fluid[n]) {
+        bflag[n] = 0;
+        return;
+    }
+    uint32_t m = 0;
+#pragma unroll
+    for (int d = 0; d < Q; d++) {
+        const int64_t a = i + Lat<Q>::cx[d], b = j + Lat<Q>::cy[d], c = k + Lat<Q>::cz[d];
+        const bool inside = a >= 0 && a < nx && b >= 0 && b < ny && c >= 0 && c < nz;
+        if (!inside || !fluid[a + nxp * (b + ny * c)]) m |= 1u << d;
+    }
+    bflag[n] = m != 0;
+}
+
+// IdentifyNextToWallLattices (Subdomain.cpp:201-218): interior fluid nodes linked to a wall node.
+// Interior nodes have all Q neighbours inside the box.
+template <int Q>
+__global__ __launch_bounds__(256) void k_ntw(uint8_t* kind, int64_t nxp, int64_t ny, int64_t nz) {
+    const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (n >= nxp * ny * nz) return;
+    if (kind[n] != K_FLUID) return;
+    bool near_wall = false;
+#pragma unroll
+    for (int d = 0; d < Q; d++) {
+        const int64_t o = Lat<Q>::cx[d] + nxp * (Lat<Q>::cy[d] + ny * Lat<Q>::cz[d]);
+        near_wall |= kind[n + o] == K_WALL;
+    }
+    if (near_wall) kind[n] = K_NTW;
+}
+
+// Per-triangle quantities of calculateBoundaryNormals (Subdomain.cpp:318-341): unit
+// cross-product normal, area, centroid.  prep planes: nx, ny, nz, area, cx, cy, cz.
+__global__ __launch_bounds__(256) void k_tri_prep(const double* tri, int64_t ntri, double* prep) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= ntri) return;
+    const double* T = tri + 12 * t;
+    double ux = T[3] - T[0], uy = T[4] - T[1], uz = T[5] - T[2];
+    double vx = T[6] - T[0], vy = T[7] - T[1], vz = T[8] - T[2];
+    double nx = uy * vz - uz * vy;
+    double ny = uz * vx - ux * vz;
+    double nz = ux * vy - uy * vx;
+    double norm = sqrt(nx * nx + ny * ny + nz * nz);
+    if (norm > 0) {
+        nx /= norm;
+        ny /= norm;
+        nz /= norm;
+    }
+    prep[t] = nx;
+    prep[ntri + t] = ny;
+    prep[2 * ntri + t] = nz;
+    prep[3 * ntri + t] = 0.5 * norm;
+    prep[4 * ntri + t] = (T[0] + T[3] + T[6]) / 3.0;
+    prep[5 * ntri + t] = (T[1] + T[4] + T[7]) / 3.0;
+    prep[6 * ntri + t] = (T[2] + T[5] + T[8]) / 3.0;
+}
+
+// Normal of each listed node: sum over all triangles of unit normal * area / distance^2, then
+// normalised (Subdomain.cpp:343-368).  One block per node, FP64 block reduction.
+__global__ __launch_bounds__(256) void k_normals(const double* prep, int64_t ntri, const double* xyz,
+                                                int64_t n, double* nrm) {
+    const int64_t w = blockIdx.x;
+    const double x = xyz[w], y = xyz[n + w], z = xyz[2 * n + w];
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    for (int64_t t = threadIdx.x; t < ntri; t += 256) {
+        double cX = prep[4 * ntri + t], cY = prep[5 * ntri + t], cZ = prep[6 * ntri + t];
+        double distance = sqrt((x - cX) * (x - cX) + (y - cY) * (y - cY) + (z - cZ) * (z - cZ));
+        double weight = prep[3 * ntri + t] / (distance * distance);
+        a0 += prep[t] * weight;
+        a1 += prep[ntri + t] * weight;
+        a2 += prep[2 * ntri + t] * weight;
+    }
+    __shared__ double s[3][256];
+    s[0][threadIdx.x] = a0;
+    s[1][threadIdx.x] = a1;
+    s[2][threadIdx.x] = a2;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h)
+            for (int c = 0; c < 3; c++) s[c][threadIdx.x] += s[c][threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double n0 = s[0][0], n1 = s[1][0], n2 = s[2][0];
+        double norm = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+        if (norm > 0) {
+            n0 /= norm;
+            n1 /= norm;
+            n2 /= norm;
+        }
+        nrm[w] = n0;
+        nrm[n + w] = n1;
+        nrm[2 * n + w] = n2;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_voxelize(const VoxParams& p, hipStream_t s) {
+    const int64_t cols = (int64_t)p.nx * p.ny;
+    if (cols <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_voxelize, dim3((unsigned)cols), dim3(64), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_boundary(int q, const uint8_t* fluid_pad, int64_t nxp, int64_t ny, int64_t nz,
+                           int64_t nx, uint8_t* bflag, hipStream_t s) {
+    const int64_t total = nxp * ny * nz;
+    const unsigned g = (unsigned)((total + 255) / 256);
+    if (q == 19) hipLaunchKernelGGL(k_boundary<19>, dim3(g), dim3(256), 0, s, fluid_pad, nxp, ny, nz, nx, bflag);
+    else hipLaunchKernelGGL(k_boundary<27>, dim3(g), dim3(256), 0, s, fluid_pad, nxp, ny, nz, nx, bflag);
+    return hipGetLastError();
+}
+
+hipError_t launch_ntw(int q, uint8_t* kind, int64_t nxp, int64_t ny, int64_t nz, hipStream_t s) {
+    const int64_t total = nxp * ny * nz;
+    const unsigned g = (unsigned)((total + 255) / 256);
+    if (q == 19) hipLaunchKernelGGL(k_ntw<19>, dim3(g), dim3(256), 0, s, kind, nxp, ny, nz);
+    else hipLaunchKernelGGL(k_ntw<27>, dim3(g), dim3(256), 0, s, kind, nxp, ny, nz);
+    return hipGetLastError();
+}
+
+hipError_t launch_tri_prep(const double* tri, int64_t ntri, double* prep, hipStream_t s) {
+    if (ntri <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_tri_prep, dim3((unsigned)((ntri + 255) / 256)), dim3(256), 0, s, tri, ntri, prep);
+    return hipGetLastError();
+}
+
+hipError_t launch_normals(const double* prep, int64_t ntri, const double* xyz, int64_t n, double* nrm,
+                          hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_normals, dim3((unsigned)n), dim3(256), 0, s, prep, ntri, xyz, n, nrm);
+    return hipGetLastError();
+}
+
+}  // namespace pvlbm

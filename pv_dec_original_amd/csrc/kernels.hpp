@@ -1,0 +1,82 @@
+// kernels.hpp — launch parameter blocks and launcher declarations shared by the host C-ABI code
+// (capi.cpp) and the gfx950 kernels (k_*.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pvlbm {
+
+// Step-kernel flags
+enum StepFlags : int {
+    F_WSS = 1,     // evaluate WSS and its accumulators on next-to-wall nodes
+    F_MACRO = 2,   // write rho/u of this launch's collision to the macro planes
+    F_REDUCE = 4,  // write FP64 block partial sums (mass, inlet/outlet flux, non-finite count)
+    F_INIT = 8,    // no gather/closure: start from f = w or f = f_eq(init_rho, init_u)
+};
+
+// One launch of the fused step kernel: the stream of iteration n-1 (pull from A), its closures
+// and WSS, then the collision of iteration n into B.
+struct StepParams {
+    const void* A;      // post-collision populations of the previous launch, plane d at A + d*ns
+    void* B;            // post-collision populations written by this launch
+    int64_t ns;         // elements per direction plane (padded node count + guard)
+    int64_t sy, sz;     // index strides of y and z (x stride 1)
+    const uint8_t* kind;   // enum Kind per padded node
+    const int32_t* aux;    // side-table slot of boundary / next-to-wall nodes
+    const int32_t* tiles;  // active 64-node tiles
+    int32_t ntiles;
+    int32_t flags;
+    // boundary table (inlet, outlet and wall nodes)
+    const uint32_t* bmask;  // bit d set: neighbour x + c_d is not a node
+    const uint8_t* bclose;  // enum BClose
+    const double* bprof;    // inlet velocity profile
+    const double* brho;     // prescribed outlet density
+    void* P;                // stale populations [Q][pst] (arithmetic type)
+    int64_t pst;
+    // next-to-wall table
+    void* M;                // previous (rho, ux, uy, uz) [4][mst] (arithmetic type)
+    int64_t mst;
+    const double* nrm;      // normals [3][mst]
+    double* wss;            // [mst]
+    double* tau3;           // [3][mst]
+    double* acc;            // [4][mst]
+    // scalars
+    double omega;           // 1/tau
+    double scale;           // inlet velocity scale of the iteration being closed
+    double C;               // WSS constant 1060 dt^2 / (h^2 1e-6)
+    // outputs
+    double* mrho;           // macro planes [4][ns] (rho, ux, uy, uz), F_MACRO
+    double* partials;       // [gridDim][4], F_REDUCE
+    unsigned long long* gate;  // stream-gate events
+    const double* init_rho; // F_INIT warm start (padded layout), NULL = rest
+    const double* init_u;   // [3][ns]
+};
+
+// Voxeliser launch: one wave per (i, j) column, candidate triangles in CSR form.
+struct VoxParams {
+    const double* tri;       // [T][12]
+    const int64_t* col_off;  // [nx*ny + 1]
+    const int32_t* col_tri;  // candidate triangle ids
+    double ox, oy, oz, h;
+    double bb[6];
+    int32_t nx, ny, nz;
+    uint8_t* fluid;          // nx*ny*nz, x fastest
+};
+
+// storage codes match enum pvlbm_storage
+hipError_t launch_step(int q, int storage, const StepParams& p, int nblocks, hipStream_t s);
+hipError_t launch_reduce_final(const double* partials, int nparts, double* out4, hipStream_t s);
+hipError_t launch_voxelize(const VoxParams& p, hipStream_t s);
+// boundary flags: per padded node, 1 if fluid and a Q neighbour is missing
+hipError_t launch_boundary(int q, const uint8_t* fluid_pad, int64_t nxp, int64_t ny, int64_t nz,
+                           int64_t nx, uint8_t* bflag, hipStream_t s);
+// next-to-wall marking: FLUID nodes with a WALL Q-neighbour become NTW
+hipError_t launch_ntw(int q, uint8_t* kind, int64_t nxp, int64_t ny, int64_t nz, hipStream_t s);
+// per-triangle unit normal, area, centroid (Subdomain.cpp:244-267)
+hipError_t launch_tri_prep(const double* tri, int64_t ntri, double* prep /*[8][ntri]*/, hipStream_t s);
+// area/d^2-weighted normal of each listed node (Subdomain.cpp:301-369)
+hipError_t launch_normals(const double* prep, int64_t ntri, const double* xyz /*[3][n]*/, int64_t n,
+                          double* nrm /*[3][n]*/, hipStream_t s);
+
+}  // namespace pvlbm

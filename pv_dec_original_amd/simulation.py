@@ -1,0 +1,267 @@
+"""SubDomain and Simulation over the device-resident engine of libpvlbm.so.
+
+``SubDomain(geometry, faces)`` replaces the reference's SubDomain(Geometry&) (Subdomain.cpp:3-32):
+boundary classification by face specs, node kinds, links, next-to-wall set, normals and the f = w
+start.  On one GPU the MPI sub-boxes and halos collapse to a single domain.
+
+``Simulation(subdomain, tau, pressure, del_t)`` replaces Simulation(SubDomain&, tau, pressure,
+del_t) (Simulation.h:26).  The reference runs its whole time loop inside the constructor
+(Simulation.cpp:13); here that happens with ``iterate=True``, and ``step`` advances any number of
+iterations of the loop body (Simulation.cpp:577-586).
+"""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+from . import _native as N
+
+BIG = 1e300
+
+
+def face(role, closure, axis, plane, tol=1e-2, lo=(-BIG, -BIG), hi=(BIG, BIG), rho=1.0):
+    """A face spec: boundary nodes with |c[axis] - plane| <= tol inside the open window
+    lo < (other two coordinates) < hi (ComplexGeometry.cpp:1380-1407 generalised)."""
+    f = N.Face()
+    f.role, f.closure, f.axis = role, closure, axis
+    f.plane, f.tol, f.rho = float(plane), float(tol), float(rho)
+    f.lo[0], f.lo[1] = lo
+    f.hi[0], f.hi[1] = hi
+    return f
+
+
+def pv_faces():
+    """The reference's hard-coded PV.stl planes (ComplexGeometry.cpp:1378-1408): inlet on
+    x = -16.9538, outlet 1 on y = -170.016 (y-max closure), outlets 2, 3 on z = -110 and z = -123
+    (z-min closure).  The reference compares against long double literals; these are the nearest
+    doubles."""
+    return [face(N.INLET, N.INLET_XMIN, 0, -16.9538, 1e-2, (-191.0, -118.0), (-187.0, -114.0)),
+            face(N.OUTLET1, N.OUTLET_YMAX, 1, -170.016, 1e-2, (2.0, -109.0), (5.0, -106.0)),
+            face(N.OUTLET2, N.OUTLET_ZMIN, 2, -110.0, 1e-2, (-27.0, -198.0), (-25.0, -195.0)),
+            face(N.OUTLET3, N.OUTLET_ZMIN, 2, -123.0, 1e-2, (-33.0, -189.0), (-30.0, -186.0))]
+
+
+STORAGE = {"fp64": N.FP64, "fp16": N.FP16S, "bf16": N.BF16S}
+
+# reference time step (LBM.cpp:59) used when a SubDomain is built before its Simulation
+REFERENCE_DT = 8e-6
+
+
+class SubDomain:
+    """Device-resident lattice built from a Geometry (see module doc)."""
+
+    def __init__(self, geometry, faces=(), storage="fp64", tau=1.0, dt=REFERENCE_DT):
+        self.Geometry_ = geometry
+        self.m_velSet = geometry.m_velSet
+        self.q = self.m_velSet.q
+        self.storage = storage
+        self.faces = list(faces)
+        arr = (N.Face * max(1, len(self.faces)))(*self.faces)
+        g = N.Grid()
+        for a in range(3):
+            g.origin[a] = float(geometry.origin[a])
+            g.dims[a] = int(geometry.dims[a])
+        h = C.c_void_p()
+        fluid = np.ascontiguousarray(geometry.fluid, dtype=np.uint8)
+        tri = np.ascontiguousarray(geometry.globalTriangles, dtype=np.float64)
+        N.check(N.lib().pvlbm_create(N.u8ptr(fluid), C.byref(g), geometry.spacing, self.q,
+                                     STORAGE[storage], arr, len(self.faces), N.dptr(tri), len(tri),
+                                     float(tau), float(dt), C.byref(h)))
+        self.handle = h.value
+        self.nx, self.ny, self.nz = geometry.dims
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h:
+            try:
+                N.lib().pvlbm_destroy(h)
+            except Exception:
+                pass
+            self.handle = None
+
+    def _chk(self, rc):
+        N.check(rc, self.handle)
+
+    def count(self, which):
+        n = C.c_int64(0)
+        self._chk(N.lib().pvlbm_count(self.handle, which, C.byref(n)))
+        return n.value
+
+    def list(self, which):
+        out = np.zeros(self.count(which), dtype=np.int64)
+        if len(out):
+            self._chk(N.lib().pvlbm_list(self.handle, which, N.i64ptr(out)))
+        return out
+
+    # reference list names (Subdomain.h:46-53)
+    @property
+    def InletBoundaryNode_(self):
+        return self.list(N.LIST_INLET)
+
+    @property
+    def WallBoundaryNode_(self):
+        return self.list(N.LIST_WALL)
+
+    @property
+    def nextToTheWall(self):
+        return self.list(N.LIST_NTW)
+
+    def kinds(self):
+        k = np.zeros((self.nz, self.ny, self.nx), dtype=np.uint8)
+        self._chk(N.lib().pvlbm_get_kinds(self.handle, N.u8ptr(k)))
+        return k
+
+    def counts(self):
+        return {name: self.count(lid) for name, lid in
+                [("fluid", N.LIST_FLUID), ("inlet", N.LIST_INLET), ("outlet1", N.LIST_OUTLET1),
+                 ("outlet2", N.LIST_OUTLET2), ("outlet3", N.LIST_OUTLET3), ("wall", N.LIST_WALL),
+                 ("ntw", N.LIST_NTW), ("tiles", N.LIST_TILES)]}
+
+    def GetVelSet(self):
+        return self.m_velSet
+
+
+class Simulation:
+    """Time stepping, boundary closures, WSS and output of one SubDomain (see module doc)."""
+
+    def __init__(self, SubDomain_, tau, pressure=1.0, del_t=REFERENCE_DT, iterate=False, **loop):
+        self.SubDomain_ = SubDomain_
+        self.handle = SubDomain_.handle
+        self.pressure_ = pressure  # stored; the outlets use their face density (Simulation.cpp:630)
+        self.SetRelaxation(tau)
+        self.m_dt = float(del_t)
+        N.check(N.lib().pvlbm_set_dt(self.handle, self.m_dt), self.handle)
+        self.total_runtime = 0.0
+        if iterate:
+            self.Iterate(**loop)
+
+    def _chk(self, rc):
+        N.check(rc, self.handle)
+
+    def SetRelaxation(self, tau):
+        self.m_relaxation = float(tau)
+        self._chk(N.lib().pvlbm_set_tau(self.handle, self.m_relaxation))
+
+    def set_inlet_profile(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        if len(u) != self.SubDomain_.count(N.LIST_INLET):
+            raise ValueError("one value per inlet node is required")
+        self._chk(N.lib().pvlbm_set_inlet_profile(self.handle, N.dptr(u)))
+
+    def set_wss_every(self, every):
+        self._chk(N.lib().pvlbm_set_wss_every(self.handle, int(every)))
+
+    def init_fields(self, rho=None, u=None):
+        """Restart from f = f_eq(rho, u) (dense box arrays) or from rest."""
+        if rho is None:
+            self._chk(N.lib().pvlbm_init_fields(self.handle, None, None))
+        else:
+            r = np.ascontiguousarray(rho, dtype=np.float64)
+            v = np.ascontiguousarray(u, dtype=np.float64)
+            self._chk(N.lib().pvlbm_init_fields(self.handle, N.dptr(r), N.dptr(v)))
+        self.total_runtime = 0.0
+
+    def step(self, n, inlet_scale=None, timed=False):
+        """n iterations of the loop body; inlet_scale: n values (None = 1.0)."""
+        sc = None if inlet_scale is None else np.ascontiguousarray(inlet_scale, dtype=np.float64)
+        if sc is not None and len(sc) < n:
+            raise ValueError("need one inlet scale per iteration")
+        if timed:
+            ms = C.c_double(0)
+            self._chk(N.lib().pvlbm_step_timed(self.handle, int(n), N.dptr(sc), C.byref(ms)))
+            return ms.value
+        self._chk(N.lib().pvlbm_step(self.handle, int(n), N.dptr(sc)))
+        return None
+
+    @property
+    def iteration(self):
+        it = C.c_int64(0)
+        self._chk(N.lib().pvlbm_iteration(self.handle, C.byref(it)))
+        return it.value
+
+    @staticmethod
+    def getInletVelocity(t, period=0.9):
+        """The reference's pulsatile waveform (Simulation.cpp:548-563)."""
+        return N.lib().pvlbm_pulsatile_velocity(float(t), float(period))
+
+    def Iterate(self, t_end=2.7, period=0.9, output_every=100, outdir=None, chunk=100):
+        """The reference loop (Simulation.cpp:569-601): while total_runtime <= t_end, one
+        iteration with InletVelocity_ = getInletVelocity(total_runtime, period), then
+        total_runtime += dt (an accumulated sum, as the reference does).  Every output_every
+        iterations the FlowHemodynamics CSV is written when outdir is given."""
+        while self.total_runtime <= t_end:
+            scales, times = [], []
+            t = self.total_runtime
+            while len(scales) < chunk and t <= t_end:
+                scales.append(self.getInletVelocity(t, period))
+                times.append(t)
+                t += self.m_dt
+            n = len(scales)
+            it0 = self.iteration
+            # step in pieces that end on output iterations
+            done = 0
+            while done < n:
+                it = it0 + done
+                nxt = ((it // output_every) + 1) * output_every if outdir else it0 + n
+                m = min(n - done, nxt - it)
+                self.step(m, scales[done:done + m])
+                done += m
+                if outdir and (it0 + done) % output_every == 0:
+                    self.WriteOutput(it0 + done - 1, outdir)
+            self.total_runtime = t
+
+    def macros(self):
+        """(rho, u) of the current iteration, dense [nz,ny,nx] and [3,nz,ny,nx]."""
+        sd = self.SubDomain_
+        rho = np.zeros((sd.nz, sd.ny, sd.nx))
+        u = np.zeros((3, sd.nz, sd.ny, sd.nx))
+        self._chk(N.lib().pvlbm_get_macros(self.handle, N.dptr(rho), N.dptr(u)))
+        return rho, u
+
+    def wss(self):
+        """[nw, 6]: x, y, z, WSS, OSI, TAWSS per next-to-wall node (OutputFile.cpp:884-894)."""
+        nw = self.SubDomain_.count(N.LIST_NTW)
+        out = np.zeros((nw, 6))
+        if nw:
+            self._chk(N.lib().pvlbm_get_wss(self.handle, N.dptr(out)))
+        return out
+
+    def wss_raw(self):
+        nw = self.SubDomain_.count(N.LIST_NTW)
+        tau3 = np.zeros((3, nw))
+        acc = np.zeros((4, nw))
+        if nw:
+            self._chk(N.lib().pvlbm_get_wss_raw(self.handle, N.dptr(tau3), N.dptr(acc)))
+        return tau3, acc
+
+    def reduce(self):
+        """FP64 (sum rho, inlet sum rho*ux, outlet sum rho*ux, non-finite count)."""
+        out = np.zeros(4)
+        self._chk(N.lib().pvlbm_reduce(self.handle, N.dptr(out)))
+        return out
+
+    def gate_events(self):
+        n = C.c_int64(0)
+        self._chk(N.lib().pvlbm_gate_events(self.handle, C.byref(n)))
+        return n.value
+
+    def WriteOutput(self, iteration, outdir):
+        """FlowHemodynamics/<iter>_FlowHemodynamics.csv (OutputFile.cpp:867-938)."""
+        from .output import write_surface_csv
+        folder = os.path.join(outdir, "FlowHemodynamics")
+        os.makedirs(folder, exist_ok=True)
+        path = os.path.join(folder, f"{iteration}_FlowHemodynamics.csv")
+        write_surface_csv(path, self.wss())
+        return path
+
+
+def parabolic_profile(coords, center_yz, radius, u_max):
+    """u_max (1 - r^2/R^2), clipped at 0, for inlet node coordinates [n, 3]."""
+    r2 = (coords[:, 1] - center_yz[0]) ** 2 + (coords[:, 2] - center_yz[1]) ** 2
+    return np.clip(u_max * (1.0 - r2 / radius ** 2), 0.0, None)
+
+
+def womersley_scale(t, period=4000.0, mean=0.03, amp=0.02):
+    """Mean inlet velocity of the pulsatile benchmark inlet at iteration t."""
+    return mean + amp * math.sin(2.0 * math.pi * t / period)

@@ -1,0 +1,216 @@
+"""ctypes access to the CPU oracle (oracle/libpvoracle.so) for the tests and the CPU baseline.
+
+The oracle is test infrastructure: a CPU restatement of the reference solver that the HIP library
+is checked against.  Nothing in the product imports this module.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "oracle", "libpvoracle.so")
+
+_d = C.POINTER(C.c_double)
+_u8 = C.POINTER(C.c_uint8)
+_i32 = C.POINTER(C.c_int32)
+_i64 = C.POINTER(C.c_int64)
+
+_lib = None
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(_d)
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB):
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
+    L = C.CDLL(LIB)
+    sig = {
+        "orc_read_stl_ascii": (C.c_int64, [C.c_char_p, _d, C.c_int64]),
+        "orc_bbox": (None, [_d, C.c_int64, _d]),
+        "orc_grid_dims": (None, [_d, C.c_double, _i32]),
+        "orc_point_in_triangle": (C.c_int, [C.c_double, C.c_double, C.c_double, _d]),
+        "orc_count_crossings": (C.c_int, [_d, C.c_int64, _d, _d]),
+        "orc_label_points": (None, [_d, C.c_int64, _d, _i32, _d, C.c_double, _u8, C.c_int]),
+        "orc_label_points_binned": (None, [_d, C.c_int64, _d, _i32, _d, C.c_double, _u8, C.c_int]),
+        "orc_sim_create": (C.c_void_p, [_u8, _i32, _d, C.c_double, C.c_int, C.c_void_p, C.c_int, _d,
+                                        C.c_int64, C.c_double, C.c_double, C.c_int]),
+        "orc_sim_destroy": (None, [C.c_void_p]),
+        "orc_sim_counts": (None, [C.c_void_p, _i64]),
+        "orc_sim_kinds": (None, [C.c_void_p, _u8]),
+        "orc_sim_list": (C.c_int64, [C.c_void_p, C.c_int, _i64]),
+        "orc_sim_set_inlet_profile": (None, [C.c_void_p, _d]),
+        "orc_sim_set_threads": (None, [C.c_void_p, C.c_int]),
+        "orc_sim_step_push": (None, [C.c_void_p, C.c_int, _d, C.c_int]),
+        "orc_sim_step_pull": (None, [C.c_void_p, C.c_int, _d, C.c_int]),
+        "orc_sim_macros": (None, [C.c_void_p, _d, _d]),
+        "orc_sim_populations": (None, [C.c_void_p, _d]),
+        "orc_sim_set_populations": (None, [C.c_void_p, _d]),
+        "orc_sim_wss": (None, [C.c_void_p, _d, _d, _d]),
+        "orc_sim_normals": (None, [C.c_void_p, _d]),
+        "orc_sim_gate_events": (C.c_int64, [C.c_void_p]),
+        "orc_collide": (None, [C.c_int, C.c_double, _d, _d, _d, _d]),
+        "orc_zouhe": (None, [C.c_int, C.c_int, _d, C.c_double, C.c_double]),
+        "orc_pulsatile_velocity": (C.c_double, [C.c_double, C.c_double]),
+        "orc_lattice": (None, [C.c_int, _i32, _d, _i32]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def read_stl(path):
+    L = load()
+    n = L.orc_read_stl_ascii(str(path).encode(), None, 0)
+    tri = np.zeros((max(n, 0), 12))
+    if n > 0:
+        L.orc_read_stl_ascii(str(path).encode(), _dp(tri), n)
+    return tri
+
+
+def bbox(tri):
+    bb = np.zeros(6)
+    tri = np.ascontiguousarray(tri, dtype=np.float64)
+    load().orc_bbox(_dp(tri), len(tri), _dp(bb))
+    return bb
+
+
+def grid_dims(bb, h):
+    d = np.zeros(3, dtype=np.int32)
+    load().orc_grid_dims(_dp(np.ascontiguousarray(bb, dtype=np.float64)), h, d.ctypes.data_as(_i32))
+    return tuple(int(v) for v in d)
+
+
+def label_points(tri, bb, dims, origin, h, binned=False, threads=0):
+    tri = np.ascontiguousarray(tri, dtype=np.float64)
+    out = np.zeros((dims[2], dims[1], dims[0]), dtype=np.uint8)
+    fn = load().orc_label_points_binned if binned else load().orc_label_points
+    d = np.array(dims, dtype=np.int32)
+    fn(_dp(tri), len(tri), _dp(np.ascontiguousarray(bb, dtype=np.float64)), d.ctypes.data_as(_i32),
+       _dp(np.ascontiguousarray(origin, dtype=np.float64)), h, out.ctypes.data_as(_u8), threads)
+    return out
+
+
+def count_crossings(tri, start, end):
+    tri = np.ascontiguousarray(tri, dtype=np.float64).reshape(-1, 12)
+    return load().orc_count_crossings(_dp(tri), len(tri), _dp(np.asarray(start, dtype=np.float64)),
+                                      _dp(np.asarray(end, dtype=np.float64)))
+
+
+def lattice(q):
+    c = np.zeros((q, 3), dtype=np.int32)
+    w = np.zeros(q)
+    o = np.zeros(q, dtype=np.int32)
+    load().orc_lattice(q, c.ctypes.data_as(_i32), _dp(w), o.ctypes.data_as(_i32))
+    return c, w, o
+
+
+def collide(q, tau, f):
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    feq = np.zeros(q)
+    fnew = np.zeros(q)
+    out = np.zeros(4)
+    load().orc_collide(q, tau, _dp(f), _dp(feq), _dp(fnew), _dp(out))
+    return out, feq, fnew
+
+
+def zouhe(q, closure, f, ux=0.0, rho=1.0):
+    g = np.array(f, dtype=np.float64).copy()
+    load().orc_zouhe(q, closure, _dp(g), ux, rho)
+    return g
+
+
+class OracleSim:
+    """Reference-semantics simulation on the CPU (push streaming, as Simulation.cpp does)."""
+
+    def __init__(self, fluid, origin, h, q, faces, tri, tau, dt, normals=True, threads=1):
+        from pv_dec_original_amd import _native as N
+        self.L = load()
+        fluid = np.ascontiguousarray(fluid, dtype=np.uint8)
+        self.nz, self.ny, self.nx = fluid.shape
+        dims = np.array([self.nx, self.ny, self.nz], dtype=np.int32)
+        arr = (N.Face * max(1, len(faces)))(*faces)
+        tri = np.ascontiguousarray(tri, dtype=np.float64)
+        self.q = q
+        self.h = self.L.orc_sim_create(fluid.ctypes.data_as(_u8), dims.ctypes.data_as(_i32),
+                                       _dp(np.ascontiguousarray(origin, dtype=np.float64)), h, q,
+                                       C.cast(arr, C.c_void_p), len(faces), _dp(tri), len(tri), tau, dt,
+                                       1 if normals else 0)
+        if not self.h:
+            raise RuntimeError("oracle create failed")
+        self.L.orc_sim_set_threads(self.h, threads)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.orc_sim_destroy(self.h)
+            self.h = None
+
+    def counts(self):
+        out = np.zeros(7, dtype=np.int64)
+        self.L.orc_sim_counts(self.h, out.ctypes.data_as(_i64))
+        return dict(zip(["fluid", "inlet", "outlet1", "outlet2", "outlet3", "wall", "ntw"], out.tolist()))
+
+    def kinds(self):
+        k = np.zeros((self.nz, self.ny, self.nx), dtype=np.uint8)
+        self.L.orc_sim_kinds(self.h, k.ctypes.data_as(_u8))
+        return k
+
+    def list(self, which):
+        n = self.L.orc_sim_list(self.h, which, None)
+        out = np.zeros(n, dtype=np.int64)
+        if n:
+            self.L.orc_sim_list(self.h, which, out.ctypes.data_as(_i64))
+        return out
+
+    def set_inlet_profile_box(self, prof_box):
+        p = np.ascontiguousarray(prof_box, dtype=np.float64).reshape(-1)
+        self.L.orc_sim_set_inlet_profile(self.h, _dp(p))
+
+    def step(self, n, scale=None, wss_every=1, pull=False):
+        sc = None if scale is None else np.ascontiguousarray(scale, dtype=np.float64)
+        fn = self.L.orc_sim_step_pull if pull else self.L.orc_sim_step_push
+        fn(self.h, n, _dp(sc), wss_every)
+
+    def macros(self):
+        rho = np.zeros((self.nz, self.ny, self.nx))
+        u = np.zeros((3, self.nz, self.ny, self.nx))
+        self.L.orc_sim_macros(self.h, _dp(rho), _dp(u))
+        return rho, u
+
+    def populations(self):
+        n = self.counts()["fluid"]
+        f = np.zeros((n, self.q))
+        self.L.orc_sim_populations(self.h, _dp(f))
+        return f
+
+    def set_populations(self, f):
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        self.L.orc_sim_set_populations(self.h, _dp(f))
+
+    def wss(self):
+        nw = self.counts()["ntw"]
+        out = np.zeros((nw, 6))
+        tau3 = np.zeros((3, nw))
+        acc = np.zeros((4, nw))
+        if nw:
+            self.L.orc_sim_wss(self.h, _dp(out), _dp(tau3), _dp(acc))
+        return out, tau3, acc
+
+    def normals(self):
+        nw = self.counts()["ntw"]
+        n3 = np.zeros((3, nw))
+        if nw:
+            self.L.orc_sim_normals(self.h, _dp(n3))
+        return n3
+
+    def gate_events(self):
+        return self.L.orc_sim_gate_events(self.h)
