@@ -1,0 +1,181 @@
+"""Benchmark: MLUPS and HBM-roofline fraction of the fused lattice Boltzmann step on one MI355X.
+
+Default workload: the D3Q27 FP64 Y-bifurcation (parent R = 24, daughters R = 17 at +-30 deg) in a
+512 x 256 x 256 box, tau = 0.6, pulsatile inlet u(t) = 0.03 + 0.02 sin(2 pi t / 4000) times a
+parabolic profile of unit mean, pressure outlet, WSS every 100 iterations.  A "step" is one
+iteration of the reference loop body over the whole domain.  The default K = 12000 steps is three
+inlet periods.
+
+Output: one JSON line.  value = fluid-node updates per second / 1e6 (MLUPS) over the timed steps,
+inputs resident in HBM.  roofline.achieved = fluid nodes * 2 * Q * 8 bytes / average launch time.
+cpu_baseline = the CPU oracle (a restatement of the reference solver, push streaming) on a bounded
+number of steps of the same workload, single-threaded like one reference MPI rank.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def inlet_scales(it0, n, period, mean, amp):
+    t = np.arange(it0, it0 + n, dtype=np.float64)
+    return mean + amp * np.sin(2.0 * np.pi * t / period)
+
+
+def cpu_baseline(case, prof_box, q, tau, budget_s, threads):
+    """Oracle steps on the host cores: returns (MLUPS, steps, seconds)."""
+    from tests import oracle_ctypes as O
+    geo = case.geometry
+    orc = O.OracleSim(geo.fluid, geo.origin, geo.spacing, q, case.faces, geo.globalTriangles, tau,
+                      8e-6, normals=False, threads=threads)
+    orc.set_inlet_profile_box(prof_box)
+    nfl = orc.counts()["fluid"]
+    t0 = time.perf_counter()
+    orc.step(2, np.full(2, 0.03), wss_every=100)
+    dt2 = time.perf_counter() - t0
+    n = max(2, min(200, int(budget_s / max(dt2 / 2, 1e-6))))
+    t0 = time.perf_counter()
+    orc.step(n, np.full(n, 0.03), wss_every=100)
+    el = time.perf_counter() - t0
+    return nfl * n / el / 1e6, n, el
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=12000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--lattice", type=int, default=27)
+    ap.add_argument("--storage", default="fp64", choices=["fp64", "fp16", "bf16"])
+    ap.add_argument("--box", default="512,256,256")
+    ap.add_argument("--wss-every", type=int, default=100)
+    ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of oracle CPU work")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported "
+              "(the single-device engine has no multi-GPU path).", file=sys.stderr)
+        sys.exit(2)
+
+    from pv_dec_original_amd import SubDomain, Simulation
+    from pv_dec_original_amd import _native as N
+    from pv_dec_original_amd.workloads import ybif_case
+
+    if N.device_count() < 1:
+        raise SystemExit("no HIP device visible")
+    box = tuple(int(v) for v in args.box.split(","))
+    q = args.lattice
+    tau = 0.6
+    t0 = time.perf_counter()
+    case = ybif_case(q=q, box=box, tau=tau)
+    t_geo = time.perf_counter() - t0
+    log(f"geometry: {len(case.geometry.globalTriangles)} triangles, voxelised in {t_geo:.1f} s")
+    t0 = time.perf_counter()
+    sd = SubDomain(case.geometry, case.faces, storage=args.storage, tau=tau, dt=8e-6)
+    sim = Simulation(sd, tau, 1.0, 8e-6)
+    t_setup = time.perf_counter() - t0
+    counts = sd.counts()
+    nfl = counts["fluid"]
+    nbox = box[0] * box[1] * box[2]
+    log(f"setup {t_setup:.1f} s: {counts}")
+    # parabolic inlet profile of unit mean (peak 2) on the parent tube
+    prof = case.inlet_profile(sd, 2.0)
+    prof *= 1.0 / prof.mean()
+    sim.set_inlet_profile(prof)
+    sim.set_wss_every(args.wss_every)
+    period, mean, amp = 4000.0, 0.03, 0.02
+
+    sim.step(args.warmup, inlet_scales(0, args.warmup, period, mean, amp))
+    log(f"warmup {args.warmup} steps done")
+    K = args.steps
+    sc = inlet_scales(args.warmup, K, period, mean, amp)
+    w0 = time.perf_counter()
+    ms = 0.0
+    done = 0
+    chunk = 2000
+    while done < K:
+        m = min(chunk, K - done)
+        ms += sim.step(m, sc[done:done + m], timed=True)
+        done += m
+        log(f"timed {done}/{K} steps")
+    wall = time.perf_counter() - w0
+    red = sim.reduce()
+    if red[3] != 0 or not np.isfinite(red[0]):
+        raise SystemExit(f"non-finite state after the timed steps: {red}")
+    sec = ms / 1e3
+    mlups = nfl * K / sec / 1e6
+    bytes_per_flup = 2 * q * (8 if args.storage == "fp64" else 2)
+    per_launch_s = sec / K
+    achieved = nfl * bytes_per_flup / per_launch_s / 1e9
+    dtype = {"fp64": "f64", "fp16": "f16-storage/f32", "bf16": "bf16-storage/f32"}[args.storage]
+
+    cpu = None
+    if not args.no_cpu:
+        k = sd.kinds().reshape(-1)
+        pbox = np.zeros(k.size)
+        pbox[sd.list(N.LIST_INLET)] = prof
+        v, n, el = cpu_baseline(case, pbox, q, tau, args.cpu_budget, 1)
+        cpu = {"value": round(v, 3), "unit": "MLUPS", "cores": 1, "kind": "port",
+               "sample": f"{n} iterations of the same workload by the CPU oracle (push streaming, "
+                         f"-O2 -ffp-contract=off, 1 thread) in {el:.1f} s on {cpu_model()}; "
+                         "normals skipped (timing only)"}
+        log(f"cpu baseline {v:.2f} MLUPS ({n} steps)")
+
+    out = {
+        "metric": "MLUPS + % HBM roofline",
+        "value": round(mlups, 2),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms / K, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": dtype,
+        "data": "synthetic (seeded generated mesh, voxelised on the GPU)",
+        "config": {
+            "workload": f"ybif-D3Q{q}-{args.storage}-{box[0]}x{box[1]}x{box[2]}",
+            "lattice": q, "storage": args.storage, "box": list(box), "fluid_nodes": nfl,
+            "fluid_fraction": round(nfl / nbox, 4), "tau": tau,
+            "inlet": "0.03+0.02*sin(2*pi*t/4000) x parabolic(mean 1)", "wss_every": args.wss_every,
+            "lists": counts, "wall_s_timed": round(wall, 3), "setup_s": round(t_setup, 2),
+            "voxelise_s": round(t_geo, 2),
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "bytes_per_flup": bytes_per_flup,
+            "kernel": f"k_step<{q},{args.storage}>", "launch_avg_us": round(per_launch_s * 1e6, 2),
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -112,7 +112,7 @@ struct pvlbm_sim {
     int32_t* d_tiles = nullptr;
     int32_t ntiles = 0;
     char* d_pop[2] = {nullptr, nullptr};  // allocation bases
-    void* A = nullptr;                    // plane-0 element 0 of the current lattice
+    void* A = nullptr;                    // allocation base of the current lattice
     void* B = nullptr;
     int64_t nb = 0, nw = 0;
     uint32_t* d_bmask = nullptr;
@@ -163,6 +163,7 @@ struct pvlbm_sim {
         p.A = A;
         p.B = B;
         p.ns = ns;
+        p.guard = guard;
         p.sy = nxp;
         p.sz = nxp * ny;
         p.kind = d_kind;
@@ -625,8 +626,8 @@ extern "C" int pvlbm_create(const uint8_t* fluid, const pvlbm_grid* grid, double
             S->d_pop[c] = dalloc<char>(pop_bytes);
             HIPCHK(hipMemsetAsync(S->d_pop[c], 0, pop_bytes, st));
         }
-        S->A = S->d_pop[0] + S->guard * S->store_bytes();
-        S->B = S->d_pop[1] + S->guard * S->store_bytes();
+        S->A = S->d_pop[0];
+        S->B = S->d_pop[1];
         HIPCHK(hipMemcpyAsync(S->d_aux, aux.data(), sizeof(int32_t) * nsp, hipMemcpyHostToDevice, st));
         if (!tiles.empty())
             HIPCHK(hipMemcpyAsync(S->d_tiles, tiles.data(), sizeof(int32_t) * tiles.size(), hipMemcpyHostToDevice, st));

@@ -70,23 +70,26 @@ __device__ __forceinline__ T wave_sum(T v) {
     return v;
 }
 
+// Byte offset of node n inside a plane: the guard keeps n + guard - offset >= 0 for every stencil
+// offset, so the per-lane part of every address is an unsigned 32-bit byte offset and the plane
+// base stays scalar (global_load ... saddr).
+template <class S>
+__device__ __forceinline__ uint32_t node_bytes(int64_t n, int64_t guard) {
+    return (uint32_t)((n + guard) * (int64_t)sizeof(S));
+}
+
+// All Q pulls issued back to back, unconditionally: for a boundary node the upstream of a missing
+// link is outside the fluid (or the box) and reads a guard or solid slot, whose value is replaced
+// from the stale table afterwards.
 template <int Q, class ST, int D = 0>
-__device__ __forceinline__ void gather(const StepParams& p, int64_t n, uint32_t nmiss, int32_t slot,
-                                       typename ST::R (&f)[Q], bool& gate) {
+__device__ __forceinline__ void load_all(const StepParams& p, uint32_t nb, typename ST::S (&v)[Q]) {
     if constexpr (D < Q) {
-        using R = typename ST::R;
         using S = typename ST::S;
         constexpr int64_t cx = Lat<Q>::cx[D], cy = Lat<Q>::cy[D], cz = Lat<Q>::cz[D];
-        // upstream x - c_D is the neighbour in direction opp(D)
-        if ((nmiss >> Lat<Q>::opp[D]) & 1u) {
-            f[D] = static_cast<const R*>(p.P)[D * p.pst + slot];
-        } else {
-            const S* A = static_cast<const S*>(p.A) + D * p.ns;
-            const S v = A[n - (cx + cy * p.sy + cz * p.sz)];
-            f[D] = ST::load(v, Lat<Q>::w[D]);
-            if constexpr (ST::kGate) gate |= (v == 0.0);
-        }
-        gather<Q, ST, D + 1>(p, n, nmiss, slot, f, gate);
+        const int64_t offb = (cx + cy * p.sy + cz * p.sz) * (int64_t)sizeof(S);
+        const char* plane = static_cast<const char*>(p.A) + D * p.ns * (int64_t)sizeof(S);
+        v[D] = *reinterpret_cast<const S*>(plane + (uint32_t)(nb - (uint32_t)offb));
+        load_all<Q, ST, D + 1>(p, nb, v);
     }
 }
 
@@ -101,7 +104,7 @@ __device__ __forceinline__ void save_stale(const StepParams& p, uint32_t nmiss, 
 }
 
 template <int Q, class ST, int D = 0>
-__device__ __forceinline__ void collide_store(const StepParams& p, int64_t n, typename ST::R rho,
+__device__ __forceinline__ void collide_store(const StepParams& p, uint32_t nb, typename ST::R rho,
                                               const typename ST::R (&u)[3],
                                               const typename ST::R (&f)[Q], typename ST::R omega) {
     if constexpr (D < Q) {
@@ -109,30 +112,50 @@ __device__ __forceinline__ void collide_store(const StepParams& p, int64_t n, ty
         using S = typename ST::S;
         const R feq = feq_dir<Q, D>(rho, u);
         const R fo = f[D] + omega * (feq - f[D]);
-        static_cast<S*>(p.B)[D * p.ns + n] = ST::store(fo, Lat<Q>::w[D]);
-        collide_store<Q, ST, D + 1>(p, n, rho, u, f, omega);
+        char* plane = static_cast<char*>(p.B) + D * p.ns * (int64_t)sizeof(S);
+        *reinterpret_cast<S*>(plane + nb) = ST::store(fo, Lat<Q>::w[D]);
+        collide_store<Q, ST, D + 1>(p, nb, rho, u, f, omega);
     }
 }
 
-template <int Q, class ST>
+template <int Q, class ST, bool WSS>
 __device__ __forceinline__ bool node_update(const StepParams& p, double (&red)[4]) {
     using R = typename ST::R;
+    using S = typename ST::S;
     const int t = xcd_remap(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
     if (t >= p.ntiles) return false;
     const int64_t n = (int64_t)p.tiles[t] * 64 + (threadIdx.x & 63);
     const uint8_t kd = p.kind[n];
     if (kd == K_SOLID) return false;
     const bool bnd = kd >= K_INLET;
+    const bool init = (p.flags & F_INIT) != 0;
+    const uint32_t nb = node_bytes<S>(n, p.guard);
+    S v[Q];
+    // issue the pulls first; the side-table lookups below overlap with them
+    if (!init) load_all<Q, ST>(p, nb, v);
     const int32_t slot = (kd >= K_NTW) ? p.aux[n] : -1;
     const uint32_t nmiss = bnd ? p.bmask[slot] : 0u;
-    const bool init = (p.flags & F_INIT) != 0;
 
     R f[Q];
     if (!init) {
-        bool gate = false;
-        gather<Q, ST>(p, n, nmiss, slot, f, gate);
-        if (gate) atomicAdd(p.gate, 1ull);
+#pragma unroll
+        for (int d = 0; d < Q; d++) f[d] = ST::load(v[d], Lat<Q>::w[d]);
+        if constexpr (ST::kGate) {
+            // the reference streams only non-zero values (Simulation.cpp:222); a zero pull on an
+            // existing link would keep a value this scheme does not store: count it
+            bool gate = false;
+#pragma unroll
+            for (int d = 0; d < Q; d++) gate |= (v[d] == S(0)) && !((nmiss >> Lat<Q>::opp[d]) & 1u);
+            if (gate) atomicAdd(p.gate, 1ull);
+        }
         if (bnd) {
+            const R* P = static_cast<const R*>(p.P);
+            R pv[Q];
+#pragma unroll
+            for (int d = 0; d < Q; d++) pv[d] = P[d * p.pst + slot];
+#pragma unroll
+            for (int d = 0; d < Q; d++)
+                if ((nmiss >> Lat<Q>::opp[d]) & 1u) f[d] = pv[d];
             const uint8_t cl = p.bclose[slot];
             if (cl == BC_WALL) {
                 wall_rule<Q>(f, nmiss);
@@ -164,29 +187,31 @@ __device__ __forceinline__ bool node_update(const StepParams& p, double (&red)[4
     if (kd == K_NTW) {
         R* M = static_cast<R*>(p.M);
         const int64_t ms = p.mst;
-        if ((p.flags & F_WSS) && !init) {
-            R mu[3] = {M[ms + slot], M[2 * ms + slot], M[3 * ms + slot]};
-            R feq_old[Q];
-            equilibrium<Q>(M[slot], mu, feq_old);
-            R fne[Q];
+        if constexpr (WSS) {
+            if (!init) {
+                R mu[3] = {M[ms + slot], M[2 * ms + slot], M[3 * ms + slot]};
+                R feq_old[Q];
+                equilibrium<Q>(M[slot], mu, feq_old);
+                R fne[Q];
 #pragma unroll
-            for (int d = 0; d < Q; d++) fne[d] = f[d] - feq_old[d];
-            const R nv[3] = {R(p.nrm[slot]), R(p.nrm[ms + slot]), R(p.nrm[2 * ms + slot])};
-            R s0 = R(0.0), s1 = R(0.0), s2 = R(0.0);
-            wss_sum<Q, R, 0>(fne, nv, s0);
-            wss_sum<Q, R, 1>(fne, nv, s1);
-            wss_sum<Q, R, 2>(fne, nv, s2);
-            const R C = R(p.C);
-            const R t0 = C * s0, t1 = C * s1, t2 = C * s2;
-            const R W = sqrt(t0 * t0 + t1 * t1 + t2 * t2);
-            p.wss[slot] = (double)W;
-            p.tau3[slot] = (double)t0;
-            p.tau3[ms + slot] = (double)t1;
-            p.tau3[2 * ms + slot] = (double)t2;
-            p.acc[slot] += (double)t0;
-            p.acc[ms + slot] += (double)t1;
-            p.acc[2 * ms + slot] += (double)t2;
-            p.acc[3 * ms + slot] += (double)W;
+                for (int d = 0; d < Q; d++) fne[d] = f[d] - feq_old[d];
+                const R nv[3] = {R(p.nrm[slot]), R(p.nrm[ms + slot]), R(p.nrm[2 * ms + slot])};
+                R s0 = R(0.0), s1 = R(0.0), s2 = R(0.0);
+                wss_sum<Q, R, 0>(fne, nv, s0);
+                wss_sum<Q, R, 1>(fne, nv, s1);
+                wss_sum<Q, R, 2>(fne, nv, s2);
+                const R C = R(p.C);
+                const R t0 = C * s0, t1 = C * s1, t2 = C * s2;
+                const R W = sqrt(t0 * t0 + t1 * t1 + t2 * t2);
+                p.wss[slot] = (double)W;
+                p.tau3[slot] = (double)t0;
+                p.tau3[ms + slot] = (double)t1;
+                p.tau3[2 * ms + slot] = (double)t2;
+                p.acc[slot] += (double)t0;
+                p.acc[ms + slot] += (double)t1;
+                p.acc[2 * ms + slot] += (double)t2;
+                p.acc[3 * ms + slot] += (double)W;
+            }
         }
         M[slot] = rho;
         M[ms + slot] = u[0];
@@ -194,7 +219,7 @@ __device__ __forceinline__ bool node_update(const StepParams& p, double (&red)[4
         M[3 * ms + slot] = u[2];
     }
 
-    collide_store<Q, ST>(p, n, rho, u, f, R(p.omega));
+    collide_store<Q, ST>(p, nb, rho, u, f, R(p.omega));
 
     if (p.flags & F_MACRO) {
         p.mrho[n] = (double)rho;
@@ -214,10 +239,10 @@ __device__ __forceinline__ bool node_update(const StepParams& p, double (&red)[4
     return true;
 }
 
-template <int Q, class ST>
+template <int Q, class ST, bool WSS>
 __global__ __launch_bounds__(256) void k_step(const StepParams p) {
     double red[4] = {0.0, 0.0, 0.0, 0.0};
-    node_update<Q, ST>(p, red);
+    node_update<Q, ST, WSS>(p, red);
     if (p.flags & F_REDUCE) {
         __shared__ double part[4][4];
         const int wv = threadIdx.x >> 6;
@@ -252,17 +277,23 @@ __global__ __launch_bounds__(256) void k_reduce_final(const double* partials, in
 
 }  // namespace
 
+template <int Q, class ST>
+void launch_q(const StepParams& p, dim3 g, hipStream_t s) {
+    if (p.flags & F_WSS) hipLaunchKernelGGL((k_step<Q, ST, true>), g, dim3(256), 0, s, p);
+    else hipLaunchKernelGGL((k_step<Q, ST, false>), g, dim3(256), 0, s, p);
+}
+
 hipError_t launch_step(int q, int storage, const StepParams& p, int nblocks, hipStream_t s) {
     if (nblocks <= 0) return hipSuccess;
-    dim3 g(nblocks), b(256);
+    dim3 g(nblocks);
     if (q == 19) {
-        if (storage == 0) hipLaunchKernelGGL((k_step<19, StFP64>), g, b, 0, s, p);
-        else if (storage == 1) hipLaunchKernelGGL((k_step<19, StFP16>), g, b, 0, s, p);
-        else hipLaunchKernelGGL((k_step<19, StBF16>), g, b, 0, s, p);
+        if (storage == 0) launch_q<19, StFP64>(p, g, s);
+        else if (storage == 1) launch_q<19, StFP16>(p, g, s);
+        else launch_q<19, StBF16>(p, g, s);
     } else {
-        if (storage == 0) hipLaunchKernelGGL((k_step<27, StFP64>), g, b, 0, s, p);
-        else if (storage == 1) hipLaunchKernelGGL((k_step<27, StFP16>), g, b, 0, s, p);
-        else hipLaunchKernelGGL((k_step<27, StBF16>), g, b, 0, s, p);
+        if (storage == 0) launch_q<27, StFP64>(p, g, s);
+        else if (storage == 1) launch_q<27, StFP16>(p, g, s);
+        else launch_q<27, StBF16>(p, g, s);
     }
     return hipGetLastError();
 }

@@ -18,9 +18,10 @@ enum StepFlags : int {
 // One launch of the fused step kernel: the stream of iteration n-1 (pull from A), its closures
 // and WSS, then the collision of iteration n into B.
 struct StepParams {
-    const void* A;      // post-collision populations of the previous launch, plane d at A + d*ns
-    void* B;            // post-collision populations written by this launch
-    int64_t ns;         // elements per direction plane (padded node count + guard)
+    const void* A;      // post-collision populations of the previous launch (allocation base)
+    void* B;            // post-collision populations written by this launch (allocation base)
+    int64_t ns;         // elements per direction plane: guard + padded node count + guard
+    int64_t guard;      // guard elements before node 0 of each plane (> largest stencil offset)
     int64_t sy, sz;     // index strides of y and z (x stride 1)
     const uint8_t* kind;   // enum Kind per padded node
     const int32_t* aux;    // side-table slot of boundary / next-to-wall nodes
