@@ -74,6 +74,8 @@ def main():
     ap.add_argument("--wss-every", type=int, default=100)
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of oracle CPU work")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-bytes", type=int, default=1 << 30,
+                    help="size of the calibration copy kernel (0 = skip)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -133,6 +135,14 @@ def main():
     achieved = nfl * bytes_per_flup / per_launch_s / 1e9
     dtype = {"fp64": "f64", "fp16": "f16-storage/f32", "bf16": "bf16-storage/f32"}[args.storage]
 
+    copy_gbs = None
+    if args.copy_bytes:
+        import ctypes as C
+        cms = C.c_double(0)
+        N.check(N.lib().pvlbm_copy_bench(args.copy_bytes, 10, C.byref(cms)))
+        copy_gbs = 2 * (args.copy_bytes // 16 * 16) * 10 / (cms.value / 1e3) / 1e9
+        log(f"copy kernel: {copy_gbs:.0f} GB/s (read + write of {args.copy_bytes} bytes)")
+
     cpu = None
     if not args.no_cpu:
         k = sd.kinds().reshape(-1)
@@ -171,6 +181,7 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
             "bytes_per_flup": bytes_per_flup,
             "kernel": f"k_step<{q},{args.storage}>", "launch_avg_us": round(per_launch_s * 1e6, 2),
+            "copy_kernel_GBs": None if copy_gbs is None else round(copy_gbs, 1),
         },
         "cpu_baseline": cpu,
     }

@@ -213,6 +213,12 @@ double pvlbm_pulsatile_velocity(double t, double period);
 
 int pvlbm_destroy(pvlbm_handle h);
 
+/* Measurement helper (no reference counterpart): run `reps` launches of a plain streaming copy of
+ * `bytes` bytes (rounded down to 16) between two device buffers and return the elapsed
+ * milliseconds.  Its known traffic (bytes read + bytes written per launch) calibrates hardware
+ * traffic counters and gives the attainable copy bandwidth. */
+int pvlbm_copy_bench(int64_t bytes, int reps, double* ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -67,6 +67,7 @@ PVLBM_API = [
     ("pvlbm_gate_events", C.c_int, [C.c_void_p, _i64]),
     ("pvlbm_pulsatile_velocity", C.c_double, [C.c_double, C.c_double]),
     ("pvlbm_destroy", C.c_int, [C.c_void_p]),
+    ("pvlbm_copy_bench", C.c_int, [C.c_int64, C.c_int, _d]),
 ]
 
 PVGEN_API = [

@@ -109,7 +109,7 @@ struct pvlbm_sim {
     // device arrays
     uint8_t* d_kind = nullptr;
     int32_t* d_aux = nullptr;
-    int32_t* d_tiles = nullptr;
+    TileDesc* d_tdesc = nullptr;
     int32_t ntiles = 0;
     char* d_pop[2] = {nullptr, nullptr};  // allocation bases
     void* A = nullptr;                    // allocation base of the current lattice
@@ -138,7 +138,7 @@ struct pvlbm_sim {
 
     size_t store_bytes() const { return storage == PVLBM_FP64 ? 8 : 2; }
     size_t real_bytes() const { return storage == PVLBM_FP64 ? 8 : 4; }
-    int nblocks() const { return (ntiles + 3) / 4; }
+    int blocks() const { return (ntiles + 3) / 4; }
     int64_t box_of_pad(int64_t n) const {
         int64_t i = n % nxp, r = n / nxp;
         return i + nx * r;
@@ -150,7 +150,7 @@ struct pvlbm_sim {
 
     ~pvlbm_sim() {
         if (stream) (void)hipStreamSynchronize(stream);
-        void* ptrs[] = {d_kind, d_aux, d_tiles, d_pop[0], d_pop[1], d_bmask, d_bclose, d_bprof, d_brho,
+        void* ptrs[] = {d_kind, d_aux, d_tdesc, d_pop[0], d_pop[1], d_bmask, d_bclose, d_bprof, d_brho,
                         d_P, d_M, d_nrm, d_wss, d_tau3, d_acc, d_macro, d_partials, d_red, d_gate};
         for (void* p : ptrs) dfree(p);
         if (ev0) (void)hipEventDestroy(ev0);
@@ -166,10 +166,9 @@ struct pvlbm_sim {
         p.guard = guard;
         p.sy = nxp;
         p.sz = nxp * ny;
-        p.kind = d_kind;
-        p.aux = d_aux;
-        p.tiles = d_tiles;
+        p.tdesc = d_tdesc;
         p.ntiles = ntiles;
+        p.blocks = blocks();
         p.flags = flags;
         p.bmask = d_bmask;
         p.bclose = d_bclose;
@@ -196,10 +195,10 @@ struct pvlbm_sim {
         StepParams p = params(flags, scale);
         p.init_rho = init_rho;
         p.init_u = init_u;
-        HIPCHK(launch_step(q, storage, p, nblocks(), stream));
+        HIPCHK(launch_step(q, storage, p, stream));
         std::swap(A, B);
         if (flags & F_REDUCE) {
-            HIPCHK(launch_reduce_final(d_partials, nblocks(), d_red, stream));
+            HIPCHK(launch_reduce_final(d_partials, blocks(), d_red, stream));
             red_valid = true;
         }
     }
@@ -594,6 +593,32 @@ extern "C" int pvlbm_create(const uint8_t* fluid, const pvlbm_grid* grid, double
         S->nw = (int64_t)wnode.size();
         S->ntiles = (int32_t)tiles.size();
         for (int32_t t : tiles) S->lists[PVLBM_LIST_TILES].push_back(t);
+        // tile descriptors: lane masks and first side-table slots (slots follow node order)
+        std::vector<TileDesc> tdesc(tiles.size());
+        {
+            int32_t bs = 0, wsl = 0;
+            for (size_t ti = 0; ti < tiles.size(); ti++) {
+                TileDesc& D = tdesc[ti];
+                D.tile = tiles[ti];
+                D.bslot = bs;
+                D.wslot = wsl;
+                D.pad = 0;
+                D.node = D.ntw = D.bnd = 0;
+                for (int l = 0; l < 64; l++) {
+                    const uint8_t kd = kind[(size_t)tiles[ti] * 64 + l];
+                    if (kd == K_SOLID) continue;
+                    D.node |= 1ull << l;
+                    if (kd == K_NTW) {
+                        D.ntw |= 1ull << l;
+                        wsl++;
+                    } else if (kd >= K_INLET) {
+                        D.bnd |= 1ull << l;
+                        bs++;
+                    }
+                }
+            }
+            if (bs != S->nb || wsl != (int32_t)wnode.size()) fail(PVLBM_EINVAL, "internal: slot order mismatch");
+        }
         xyz.resize((size_t)(3 * S->nw));
         for (int64_t w = 0; w < S->nw; w++) {
             const int64_t n = wnode[w];
@@ -606,19 +631,19 @@ extern "C" int pvlbm_create(const uint8_t* fluid, const pvlbm_grid* grid, double
         // device tables
         const int64_t nb1 = std::max<int64_t>(S->nb, 1), nw1 = std::max<int64_t>(S->nw, 1);
         S->d_aux = dalloc<int32_t>((size_t)nsp);
-        S->d_tiles = dalloc<int32_t>(tiles.size());
+        S->d_tdesc = dalloc<TileDesc>(tdesc.size());
         S->d_bmask = dalloc<uint32_t>((size_t)nb1);
         S->d_bclose = dalloc<uint8_t>((size_t)nb1);
         S->d_bprof = dalloc<double>((size_t)nb1);
         S->d_brho = dalloc<double>((size_t)nb1);
-        S->d_P = dalloc<char>((size_t)(q * nb1) * S->real_bytes());
+        S->d_P = dalloc<char>((size_t)(q * nb1) * S->store_bytes());
         S->d_M = dalloc<char>((size_t)(4 * nw1) * S->real_bytes());
         S->d_nrm = dalloc<double>((size_t)(3 * nw1));
         S->d_wss = dalloc<double>((size_t)nw1);
         S->d_tau3 = dalloc<double>((size_t)(3 * nw1));
         S->d_acc = dalloc<double>((size_t)(4 * nw1));
         S->d_macro = dalloc<double>((size_t)(4 * S->ns));
-        S->d_partials = dalloc<double>((size_t)(4 * std::max(S->nblocks(), 1)));
+        S->d_partials = dalloc<double>((size_t)(4 * std::max(S->blocks(), 1)));
         S->d_red = dalloc<double>(4);
         S->d_gate = dalloc<unsigned long long>(1);
         const size_t pop_bytes = (size_t)q * (size_t)S->ns * S->store_bytes();
@@ -629,8 +654,8 @@ extern "C" int pvlbm_create(const uint8_t* fluid, const pvlbm_grid* grid, double
         S->A = S->d_pop[0];
         S->B = S->d_pop[1];
         HIPCHK(hipMemcpyAsync(S->d_aux, aux.data(), sizeof(int32_t) * nsp, hipMemcpyHostToDevice, st));
-        if (!tiles.empty())
-            HIPCHK(hipMemcpyAsync(S->d_tiles, tiles.data(), sizeof(int32_t) * tiles.size(), hipMemcpyHostToDevice, st));
+        if (!tdesc.empty())
+            HIPCHK(hipMemcpyAsync(S->d_tdesc, tdesc.data(), sizeof(TileDesc) * tdesc.size(), hipMemcpyHostToDevice, st));
         if (S->nb) {
             HIPCHK(hipMemcpyAsync(S->d_bmask, bmask.data(), sizeof(uint32_t) * S->nb, hipMemcpyHostToDevice, st));
             HIPCHK(hipMemcpyAsync(S->d_bclose, bclose.data(), S->nb, hipMemcpyHostToDevice, st));
@@ -639,7 +664,7 @@ extern "C" int pvlbm_create(const uint8_t* fluid, const pvlbm_grid* grid, double
         }
         HIPCHK(hipMemsetAsync(S->d_nrm, 0, sizeof(double) * 3 * nw1, st));
         HIPCHK(hipMemsetAsync(S->d_M, 0, (size_t)(4 * nw1) * S->real_bytes(), st));
-        HIPCHK(hipMemsetAsync(S->d_P, 0, (size_t)(q * nb1) * S->real_bytes(), st));
+        HIPCHK(hipMemsetAsync(S->d_P, 0, (size_t)(q * nb1) * S->store_bytes(), st));
         HIPCHK(hipMemsetAsync(S->d_macro, 0, sizeof(double) * 4 * S->ns, st));
         // normals of the next-to-wall nodes against every triangle
         if (S->nw > 0 && ntri > 0) {
@@ -920,4 +945,41 @@ extern "C" double pvlbm_pulsatile_velocity(double x, double period) {
 extern "C" int pvlbm_destroy(pvlbm_handle h) {
     delete h;
     return PVLBM_OK;
+}
+
+extern "C" int pvlbm_copy_bench(int64_t bytes, int reps, double* ms) {
+    return guarded(nullptr, [&] {
+        if (bytes < 16 || reps < 1 || !ms) fail(PVLBM_EINVAL, "bad copy arguments");
+        bytes = bytes / 16 * 16;
+        hipStream_t s;
+        HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        char *a = nullptr, *b = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        try {
+            a = dalloc<char>((size_t)bytes);
+            b = dalloc<char>((size_t)bytes);
+            HIPCHK(hipMemsetAsync(a, 1, (size_t)bytes, s));
+            HIPCHK(hipEventCreate(&e0));
+            HIPCHK(hipEventCreate(&e1));
+            HIPCHK(launch_copy(a, b, bytes, s));
+            HIPCHK(hipEventRecord(e0, s));
+            for (int r = 0; r < reps; r++) HIPCHK(launch_copy(a, b, bytes, s));
+            HIPCHK(hipEventRecord(e1, s));
+            HIPCHK(hipEventSynchronize(e1));
+            float f = 0;
+            HIPCHK(hipEventElapsedTime(&f, e0, e1));
+            *ms = f;
+        } catch (...) {
+            (void)hipStreamSynchronize(s);
+            dfree(a); dfree(b);
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            (void)hipStreamDestroy(s);
+            throw;
+        }
+        dfree(a); dfree(b);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipStreamDestroy(s);
+    });
 }

@@ -241,3 +241,26 @@ hipError_t launch_normals(const double* prep, int64_t ntri, const double* xyz, i
 }
 
 }  // namespace pvlbm
+
+namespace pvlbm {
+
+namespace {
+
+// Plain streaming copy (16-byte vectors), used only to calibrate HBM traffic counters against a
+// kernel whose bytes are known exactly.
+__global__ __launch_bounds__(256) void k_copy(const double2* __restrict__ src, double2* __restrict__ dst,
+                                             int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        dst[i] = src[i];
+}
+
+}  // namespace
+
+hipError_t launch_copy(const void* src, void* dst, int64_t bytes, hipStream_t s) {
+    const int64_t n = bytes / 16;
+    hipLaunchKernelGGL(k_copy, dim3(8192), dim3(256), 0, s, static_cast<const double2*>(src),
+                       static_cast<double2*>(dst), n);
+    return hipGetLastError();
+}
+
+}  // namespace pvlbm

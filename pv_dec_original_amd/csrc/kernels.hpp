@@ -17,23 +17,37 @@ enum StepFlags : int {
 
 // One launch of the fused step kernel: the stream of iteration n-1 (pull from A), its closures
 // and WSS, then the collision of iteration n into B.
+// One active 64-node x-segment of the padded box.  Lane l of the tile is node 64*tile + l; the
+// masks mark which lanes are lattice nodes, next-to-wall nodes and boundary (inlet/outlet/wall)
+// nodes.  Side-table slots are assigned in node order, so the slot of a lane is the first slot of
+// the tile plus the number of masked lanes below it.
+struct alignas(16) TileDesc {
+    int32_t tile;
+    int32_t bslot;   // first boundary-table slot of the tile
+    int32_t wslot;   // first next-to-wall-table slot of the tile
+    int32_t pad;
+    uint64_t node;
+    uint64_t ntw;
+    uint64_t bnd;
+};
+
 struct StepParams {
     const void* A;      // post-collision populations of the previous launch (allocation base)
     void* B;            // post-collision populations written by this launch (allocation base)
     int64_t ns;         // elements per direction plane: guard + padded node count + guard
     int64_t guard;      // guard elements before node 0 of each plane (> largest stencil offset)
     int64_t sy, sz;     // index strides of y and z (x stride 1)
-    const uint8_t* kind;   // enum Kind per padded node
-    const int32_t* aux;    // side-table slot of boundary / next-to-wall nodes
-    const int32_t* tiles;  // active 64-node tiles
+    const TileDesc* tdesc;  // active tiles
     int32_t ntiles;
+    int32_t blocks;         // launched blocks (4 tiles each)
     int32_t flags;
+    int32_t pad0;
     // boundary table (inlet, outlet and wall nodes)
     const uint32_t* bmask;  // bit d set: neighbour x + c_d is not a node
     const uint8_t* bclose;  // enum BClose
     const double* bprof;    // inlet velocity profile
     const double* brho;     // prescribed outlet density
-    void* P;                // stale populations [Q][pst] (arithmetic type)
+    void* P;                // stale populations [Q][pst] (storage type, shifted like the lattice)
     int64_t pst;
     // next-to-wall table
     void* M;                // previous (rho, ux, uy, uz) [4][mst] (arithmetic type)
@@ -48,7 +62,7 @@ struct StepParams {
     double C;               // WSS constant 1060 dt^2 / (h^2 1e-6)
     // outputs
     double* mrho;           // macro planes [4][ns] (rho, ux, uy, uz), F_MACRO
-    double* partials;       // [gridDim][4], F_REDUCE
+    double* partials;       // [blocks][4], F_REDUCE
     unsigned long long* gate;  // stream-gate events
     const double* init_rho; // F_INIT warm start (padded layout), NULL = rest
     const double* init_u;   // [3][ns]
@@ -66,7 +80,8 @@ struct VoxParams {
 };
 
 // storage codes match enum pvlbm_storage
-hipError_t launch_step(int q, int storage, const StepParams& p, int nblocks, hipStream_t s);
+// one step over all active tiles
+hipError_t launch_step(int q, int storage, const StepParams& p, hipStream_t s);
 hipError_t launch_reduce_final(const double* partials, int nparts, double* out4, hipStream_t s);
 hipError_t launch_voxelize(const VoxParams& p, hipStream_t s);
 // boundary flags: per padded node, 1 if fluid and a Q neighbour is missing
@@ -79,5 +94,8 @@ hipError_t launch_tri_prep(const double* tri, int64_t ntri, double* prep /*[8][n
 // area/d^2-weighted normal of each listed node (Subdomain.cpp:301-369)
 hipError_t launch_normals(const double* prep, int64_t ntri, const double* xyz /*[3][n]*/, int64_t n,
                           double* nrm /*[3][n]*/, hipStream_t s);
+
+// streaming copy of `bytes` (multiple of 16) for counter calibration
+hipError_t launch_copy(const void* src, void* dst, int64_t bytes, hipStream_t s);
 
 }  // namespace pvlbm
