@@ -71,6 +71,8 @@ def main():
     ap.add_argument("--lattice", type=int, default=27)
     ap.add_argument("--storage", default="fp64", choices=["fp64", "fp16", "bf16"])
     ap.add_argument("--box", default="512,256,256")
+    ap.add_argument("--workload", default="ybif", choices=["ybif", "box"],
+                    help="ybif: the Y-bifurcation; box: an all-fluid box (kernel ceiling)")
     ap.add_argument("--wss-every", type=int, default=100)
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of oracle CPU work")
     ap.add_argument("--no-cpu", action="store_true")
@@ -85,7 +87,7 @@ def main():
 
     from pv_dec_original_amd import SubDomain, Simulation
     from pv_dec_original_amd import _native as N
-    from pv_dec_original_amd.workloads import ybif_case
+    from pv_dec_original_amd.workloads import ybif_case, fluid_box_case
 
     if N.device_count() < 1:
         raise SystemExit("no HIP device visible")
@@ -93,7 +95,7 @@ def main():
     q = args.lattice
     tau = 0.6
     t0 = time.perf_counter()
-    case = ybif_case(q=q, box=box, tau=tau)
+    case = ybif_case(q=q, box=box, tau=tau) if args.workload == "ybif" else fluid_box_case(q, box, tau)
     t_geo = time.perf_counter() - t0
     log(f"geometry: {len(case.geometry.globalTriangles)} triangles, voxelised in {t_geo:.1f} s")
     t0 = time.perf_counter()
@@ -169,7 +171,7 @@ def main():
         "dtype": dtype,
         "data": "synthetic (seeded generated mesh, voxelised on the GPU)",
         "config": {
-            "workload": f"ybif-D3Q{q}-{args.storage}-{box[0]}x{box[1]}x{box[2]}",
+            "workload": f"{args.workload}-D3Q{q}-{args.storage}-{box[0]}x{box[1]}x{box[2]}",
             "lattice": q, "storage": args.storage, "box": list(box), "fluid_nodes": nfl,
             "fluid_fraction": round(nfl / nbox, 4), "tau": tau,
             "inlet": "0.03+0.02*sin(2*pi*t/4000) x parabolic(mean 1)", "wss_every": args.wss_every,

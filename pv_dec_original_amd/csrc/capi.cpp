@@ -103,7 +103,9 @@ struct pvlbm_sim {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int q = 19, storage = PVLBM_FP64;
-    int64_t nx = 0, ny = 0, nz = 0, nxp = 0, nsp = 0, guard = 0, ns = 0;
+    // padded box: nxp >= nx + 1 (multiple of 64) and nyp = ny + 1, so that every out-of-box
+    // neighbour position of a node is a non-node slot (the home of a stale population)
+    int64_t nx = 0, ny = 0, nz = 0, nxp = 0, nyp = 0, nsp = 0, guard = 0, ns = 0;
     double origin[3] = {0, 0, 0};
     double h = 1, tau = 1, dt = 1, C = 0;
     // device arrays
@@ -119,7 +121,6 @@ struct pvlbm_sim {
     uint8_t* d_bclose = nullptr;
     double* d_bprof = nullptr;
     double* d_brho = nullptr;
-    void* d_P = nullptr;
     void* d_M = nullptr;
     double* d_nrm = nullptr;
     double* d_wss = nullptr;
@@ -139,19 +140,18 @@ struct pvlbm_sim {
     size_t store_bytes() const { return storage == PVLBM_FP64 ? 8 : 2; }
     size_t real_bytes() const { return storage == PVLBM_FP64 ? 8 : 4; }
     int blocks() const { return (ntiles + 3) / 4; }
+    // padded row index of box row r = j + ny*k
+    int64_t prow(int64_t r) const { return r % ny + nyp * (r / ny); }
     int64_t box_of_pad(int64_t n) const {
-        int64_t i = n % nxp, r = n / nxp;
-        return i + nx * r;
+        int64_t i = n % nxp, pr = n / nxp;
+        return i + nx * (pr % nyp + ny * (pr / nyp));
     }
-    int64_t pad_of_box(int64_t b) const {
-        int64_t i = b % nx, r = b / nx;
-        return i + nxp * r;
-    }
+    int64_t pad_of_box(int64_t b) const { return b % nx + nxp * prow(b / nx); }
 
     ~pvlbm_sim() {
         if (stream) (void)hipStreamSynchronize(stream);
         void* ptrs[] = {d_kind, d_aux, d_tdesc, d_pop[0], d_pop[1], d_bmask, d_bclose, d_bprof, d_brho,
-                        d_P, d_M, d_nrm, d_wss, d_tau3, d_acc, d_macro, d_partials, d_red, d_gate};
+                        d_M, d_nrm, d_wss, d_tau3, d_acc, d_macro, d_partials, d_red, d_gate};
         for (void* p : ptrs) dfree(p);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
@@ -165,7 +165,7 @@ struct pvlbm_sim {
         p.ns = ns;
         p.guard = guard;
         p.sy = nxp;
-        p.sz = nxp * ny;
+        p.sz = nxp * nyp;
         p.tdesc = d_tdesc;
         p.ntiles = ntiles;
         p.blocks = blocks();
@@ -174,8 +174,6 @@ struct pvlbm_sim {
         p.bclose = d_bclose;
         p.bprof = d_bprof;
         p.brho = d_brho;
-        p.P = d_P;
-        p.pst = std::max<int64_t>(nb, 1);
         p.M = d_M;
         p.mst = std::max<int64_t>(nw, 1);
         p.nrm = d_nrm;
@@ -462,15 +460,16 @@ extern "C" int pvlbm_create(const uint8_t* fluid, const pvlbm_grid* grid, double
         }
         const int64_t nx = grid->dims[0], ny = grid->dims[1], nz = grid->dims[2];
         if (nx <= 0 || ny <= 0 || nz <= 0) fail(PVLBM_EINVAL, "empty grid");
-        const int64_t nxp = (nx + 63) / 64 * 64;
-        const int64_t nsp = nxp * ny * nz;
+        const int64_t nxp = (nx + 1 + 63) / 64 * 64;
+        const int64_t nyp = ny + 1;
+        const int64_t nsp = nxp * nyp * nz;
         if (nsp / 64 > (int64_t)INT32_MAX) fail(PVLBM_EINVAL, "grid too large");
 
         S = new pvlbm_sim();
         S->q = lattice;
         S->storage = storage;
-        S->nx = nx; S->ny = ny; S->nz = nz; S->nxp = nxp; S->nsp = nsp;
-        S->guard = nxp * ny + nxp + 64;
+        S->nx = nx; S->ny = ny; S->nz = nz; S->nxp = nxp; S->nyp = nyp; S->nsp = nsp;
+        S->guard = nxp * nyp + nxp + 64;
         S->ns = nsp + 2 * S->guard;
         for (int a = 0; a < 3; a++) S->origin[a] = grid->origin[a];
         S->h = h; S->tau = tau; S->dt = dt;
@@ -492,14 +491,14 @@ extern "C" int pvlbm_create(const uint8_t* fluid, const pvlbm_grid* grid, double
 
         // padded fluid flags -> boundary flags on the GPU
         std::vector<uint8_t> fpad((size_t)nsp, 0);
-        for (int64_t r = 0; r < ny * nz; r++) std::memcpy(&fpad[r * nxp], fluid + r * nx, (size_t)nx);
+        for (int64_t r = 0; r < ny * nz; r++) std::memcpy(&fpad[S->prow(r) * nxp], fluid + r * nx, (size_t)nx);
         uint8_t* d_fluid = dalloc<uint8_t>((size_t)nsp);
         uint8_t* d_bflag = nullptr;
         std::vector<uint8_t> bflag((size_t)nsp);
         try {
             d_bflag = dalloc<uint8_t>((size_t)nsp);
             HIPCHK(hipMemcpyAsync(d_fluid, fpad.data(), (size_t)nsp, hipMemcpyHostToDevice, st));
-            HIPCHK(launch_boundary(q, d_fluid, nxp, ny, nz, nx, d_bflag, st));
+            HIPCHK(launch_boundary(q, d_fluid, nxp, nyp, nx, ny, nz, d_bflag, st));
             HIPCHK(hipMemcpyAsync(bflag.data(), d_bflag, (size_t)nsp, hipMemcpyDeviceToHost, st));
             HIPCHK(hipStreamSynchronize(st));
         } catch (...) {
@@ -521,7 +520,7 @@ extern "C" int pvlbm_create(const uint8_t* fluid, const pvlbm_grid* grid, double
             if (!fpad[n]) continue;
             kind[n] = K_FLUID;
             if (!bflag[n]) continue;
-            const int64_t i = n % nxp, j = (n / nxp) % ny, k = n / (nxp * ny);
+            const int64_t i = n % nxp, j = (n / nxp) % nyp, k = n / (nxp * nyp);
             long double c3[3] = {(long double)(grid->origin[0] + (double)i * h),
                                  (long double)(grid->origin[1] + (double)j * h),
                                  (long double)(grid->origin[2] + (double)k * h)};
@@ -536,7 +535,7 @@ extern "C" int pvlbm_create(const uint8_t* fluid, const pvlbm_grid* grid, double
             for (int d = 0; d < q; d++) {
                 int64_t a = i + cx[d], b = j + cy[d], c = k + cz[d];
                 bool inside = a >= 0 && a < nx && b >= 0 && b < ny && c >= 0 && c < nz;
-                if (!inside || !fpad[a + nxp * (b + ny * c)]) m |= 1u << d;
+                if (!inside || !fpad[a + nxp * (b + nyp * c)]) m |= 1u << d;
             }
             aux[n] = (int32_t)bnode.size();
             bnode.push_back(n);
@@ -561,7 +560,7 @@ extern "C" int pvlbm_create(const uint8_t* fluid, const pvlbm_grid* grid, double
         // next-to-wall marking on the GPU
         S->d_kind = dalloc<uint8_t>((size_t)nsp);
         HIPCHK(hipMemcpyAsync(S->d_kind, kind.data(), (size_t)nsp, hipMemcpyHostToDevice, st));
-        HIPCHK(launch_ntw(q, S->d_kind, nxp, ny, nz, st));
+        HIPCHK(launch_ntw(q, S->d_kind, nxp, nyp, nz, st));
         HIPCHK(hipMemcpyAsync(kind.data(), S->d_kind, (size_t)nsp, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
 
@@ -622,7 +621,7 @@ extern "C" int pvlbm_create(const uint8_t* fluid, const pvlbm_grid* grid, double
         xyz.resize((size_t)(3 * S->nw));
         for (int64_t w = 0; w < S->nw; w++) {
             const int64_t n = wnode[w];
-            const int64_t i = n % nxp, j = (n / nxp) % ny, k = n / (nxp * ny);
+            const int64_t i = n % nxp, j = (n / nxp) % nyp, k = n / (nxp * nyp);
             xyz[w] = grid->origin[0] + (double)i * h;
             xyz[S->nw + w] = grid->origin[1] + (double)j * h;
             xyz[2 * S->nw + w] = grid->origin[2] + (double)k * h;
@@ -636,7 +635,6 @@ extern "C" int pvlbm_create(const uint8_t* fluid, const pvlbm_grid* grid, double
         S->d_bclose = dalloc<uint8_t>((size_t)nb1);
         S->d_bprof = dalloc<double>((size_t)nb1);
         S->d_brho = dalloc<double>((size_t)nb1);
-        S->d_P = dalloc<char>((size_t)(q * nb1) * S->store_bytes());
         S->d_M = dalloc<char>((size_t)(4 * nw1) * S->real_bytes());
         S->d_nrm = dalloc<double>((size_t)(3 * nw1));
         S->d_wss = dalloc<double>((size_t)nw1);
@@ -664,7 +662,6 @@ extern "C" int pvlbm_create(const uint8_t* fluid, const pvlbm_grid* grid, double
         }
         HIPCHK(hipMemsetAsync(S->d_nrm, 0, sizeof(double) * 3 * nw1, st));
         HIPCHK(hipMemsetAsync(S->d_M, 0, (size_t)(4 * nw1) * S->real_bytes(), st));
-        HIPCHK(hipMemsetAsync(S->d_P, 0, (size_t)(q * nb1) * S->store_bytes(), st));
         HIPCHK(hipMemsetAsync(S->d_macro, 0, sizeof(double) * 4 * S->ns, st));
         // normals of the next-to-wall nodes against every triangle
         if (S->nw > 0 && ntri > 0) {
@@ -742,7 +739,8 @@ extern "C" int pvlbm_get_kinds(pvlbm_handle h, uint8_t* kinds) {
         std::vector<uint8_t> k((size_t)h->nsp);
         HIPCHK(hipMemcpyAsync(k.data(), h->d_kind, (size_t)h->nsp, hipMemcpyDeviceToHost, h->stream));
         HIPCHK(hipStreamSynchronize(h->stream));
-        for (int64_t r = 0; r < h->ny * h->nz; r++) std::memcpy(kinds + r * h->nx, &k[r * h->nxp], (size_t)h->nx);
+        for (int64_t r = 0; r < h->ny * h->nz; r++)
+            std::memcpy(kinds + r * h->nx, &k[h->prow(r) * h->nxp], (size_t)h->nx);
     });
 }
 
@@ -769,9 +767,10 @@ extern "C" int pvlbm_init_fields(pvlbm_handle h, const double* rho, const double
                 // padded [4][ns] staging: rho plane then three velocity planes
                 std::vector<double> pad((size_t)(4 * h->ns), 0.0);
                 for (int64_t r = 0; r < h->ny * h->nz; r++) {
-                    std::memcpy(&pad[r * h->nxp], rho + r * h->nx, sizeof(double) * h->nx);
+                    const int64_t pr = h->prow(r) * h->nxp;
+                    std::memcpy(&pad[pr], rho + r * h->nx, sizeof(double) * h->nx);
                     for (int c = 0; c < 3; c++)
-                        std::memcpy(&pad[(c + 1) * h->ns + r * h->nxp], u + c * h->nx * h->ny * h->nz + r * h->nx,
+                        std::memcpy(&pad[(c + 1) * h->ns + pr], u + c * h->nx * h->ny * h->nz + r * h->nx,
                                     sizeof(double) * h->nx);
                 }
                 d_in = dalloc<double>(pad.size());
@@ -848,7 +847,7 @@ extern "C" int pvlbm_get_macros(pvlbm_handle h, double* rho, double* u) {
         const int64_t nbox = h->nx * h->ny * h->nz;
         for (int64_t r = 0; r < h->ny * h->nz; r++)
             for (int64_t i = 0; i < h->nx; i++) {
-                const int64_t n = r * h->nxp + i, b = r * h->nx + i;
+                const int64_t n = h->prow(r) * h->nxp + i, b = r * h->nx + i;
                 const bool node = k[n] != K_SOLID;
                 if (rho) rho[b] = node ? m[n] : 0.0;
                 if (u)

@@ -100,12 +100,12 @@ __global__ __launch_bounds__(64) void k_voxelize(const VoxParams p) {
 // findBoundaryPoints (ComplexGeometry.cpp:1259-1300) over the Q directions: a fluid node is a
 // boundary node when a Q neighbour is outside the box or not fluid.
 template <int Q>
-__global__ __launch_bounds__(256) void k_boundary(const uint8_t* fluid, int64_t nxp, int64_t ny,
-                                                 int64_t nz, int64_t nx, uint8_t* bflag) {
+__global__ __launch_bounds__(256) void k_boundary(const uint8_t* fluid, int64_t nxp, int64_t nyp,
+                                                 int64_t nx, int64_t ny, int64_t nz, uint8_t* bflag) {
     const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const int64_t total = nxp * ny * nz;
+    const int64_t total = nxp * nyp * nz;
     if (n >= total) return;
-    const int64_t i = n % nxp, j = (n / nxp) % ny, k = n / (nxp * ny);
+    const int64_t i = n % nxp, j = (n / nxp) % nyp, k = n / (nxp * nyp);
     if (!fluid[n]) {
         bflag[n] = 0;
         return;
@@ -115,7 +115,7 @@ __global__ __launch_bounds__(256) void k_boundary(const uint8_t* fluid, int64_t 
     for (int d = 0; d < Q; d++) {
         const int64_t a = i + Lat<Q>::cx[d], b = j + Lat<Q>::cy[d], c = k + Lat<Q>::cz[d];
         const bool inside = a >= 0 && a < nx && b >= 0 && b < ny && c >= 0 && c < nz;
-        if (!inside || !fluid[a + nxp * (b + ny * c)]) m |= 1u << d;
+        if (!inside || !fluid[a + nxp * (b + nyp * c)]) m |= 1u << d;
     }
     bflag[n] = m != 0;
 }
@@ -123,14 +123,14 @@ __global__ __launch_bounds__(256) void k_boundary(const uint8_t* fluid, int64_t 
 // IdentifyNextToWallLattices (Subdomain.cpp:201-218): interior fluid nodes linked to a wall node.
 // Interior nodes have all Q neighbours inside the box.
 template <int Q>
-__global__ __launch_bounds__(256) void k_ntw(uint8_t* kind, int64_t nxp, int64_t ny, int64_t nz) {
+__global__ __launch_bounds__(256) void k_ntw(uint8_t* kind, int64_t nxp, int64_t nyp, int64_t nz) {
     const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (n >= nxp * ny * nz) return;
+    if (n >= nxp * nyp * nz) return;
     if (kind[n] != K_FLUID) return;
     bool near_wall = false;
 #pragma unroll
     for (int d = 0; d < Q; d++) {
-        const int64_t o = Lat<Q>::cx[d] + nxp * (Lat<Q>::cy[d] + ny * Lat<Q>::cz[d]);
+        const int64_t o = Lat<Q>::cx[d] + nxp * (Lat<Q>::cy[d] + nyp * Lat<Q>::cz[d]);
         near_wall |= kind[n + o] == K_WALL;
     }
     if (near_wall) kind[n] = K_NTW;
@@ -210,20 +210,20 @@ hipError_t launch_voxelize(const VoxParams& p, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_boundary(int q, const uint8_t* fluid_pad, int64_t nxp, int64_t ny, int64_t nz,
-                           int64_t nx, uint8_t* bflag, hipStream_t s) {
-    const int64_t total = nxp * ny * nz;
+hipError_t launch_boundary(int q, const uint8_t* fluid_pad, int64_t nxp, int64_t nyp, int64_t nx,
+                           int64_t ny, int64_t nz, uint8_t* bflag, hipStream_t s) {
+    const int64_t total = nxp * nyp * nz;
     const unsigned g = (unsigned)((total + 255) / 256);
-    if (q == 19) hipLaunchKernelGGL(k_boundary<19>, dim3(g), dim3(256), 0, s, fluid_pad, nxp, ny, nz, nx, bflag);
-    else hipLaunchKernelGGL(k_boundary<27>, dim3(g), dim3(256), 0, s, fluid_pad, nxp, ny, nz, nx, bflag);
+    if (q == 19) hipLaunchKernelGGL(k_boundary<19>, dim3(g), dim3(256), 0, s, fluid_pad, nxp, nyp, nx, ny, nz, bflag);
+    else hipLaunchKernelGGL(k_boundary<27>, dim3(g), dim3(256), 0, s, fluid_pad, nxp, nyp, nx, ny, nz, bflag);
     return hipGetLastError();
 }
 
-hipError_t launch_ntw(int q, uint8_t* kind, int64_t nxp, int64_t ny, int64_t nz, hipStream_t s) {
-    const int64_t total = nxp * ny * nz;
+hipError_t launch_ntw(int q, uint8_t* kind, int64_t nxp, int64_t nyp, int64_t nz, hipStream_t s) {
+    const int64_t total = nxp * nyp * nz;
     const unsigned g = (unsigned)((total + 255) / 256);
-    if (q == 19) hipLaunchKernelGGL(k_ntw<19>, dim3(g), dim3(256), 0, s, kind, nxp, ny, nz);
-    else hipLaunchKernelGGL(k_ntw<27>, dim3(g), dim3(256), 0, s, kind, nxp, ny, nz);
+    if (q == 19) hipLaunchKernelGGL(k_ntw<19>, dim3(g), dim3(256), 0, s, kind, nxp, nyp, nz);
+    else hipLaunchKernelGGL(k_ntw<27>, dim3(g), dim3(256), 0, s, kind, nxp, nyp, nz);
     return hipGetLastError();
 }
 

@@ -3,18 +3,19 @@
 // One launch = the streaming and boundary closures of reference iteration n-1 followed by the
 // collision of iteration n (Simulation::Iterate body, Simulation.cpp:577-586, regrouped): every
 // node pulls its populations from the post-collision lattice A and writes its own post-collision
-// populations to B.  Boundary nodes take the populations of links with no upstream node from
-// their stale table P (the reference keeps the old population there, Simulation.cpp:218-224) and
-// apply their closure (Zou-He or the wall rule); next-to-wall nodes evaluate WSS against the
-// equilibrium of their previous macros M (Simulation.cpp:880-898).
+// populations to B.  A link with no upstream node keeps its previous post-closure value (the
+// reference does not stream into it, Simulation.cpp:218-224); that value is parked in the lattice
+// slot the link pulls from, which is not a node, so every lane does the same branch-free gather.
+// Boundary nodes then apply their closure (Zou-He or the wall rule); next-to-wall nodes evaluate
+// WSS against the equilibrium of their previous macros M (Simulation.cpp:880-898).
 //
-// Work unit: one wave64 per 64-node x-segment ("tile") of the box, which is padded in x to a
-// multiple of 64.  A tile descriptor carries three 64-bit lane masks (lattice node, next-to-wall,
-// boundary) and the first side-table slot of each kind, so a lane finds its slot with mbcnt and
-// no per-lane lookup; tiles without boundary lanes (most of them) take the branch-free interior
-// path, a wave-uniform choice.  Each pull is one contiguous 512-byte (FP64) / 128-byte (16-bit)
-// run of a direction plane shifted by the stencil offset; every plane has a guard before and after
-// so that all pull addresses are in bounds and the per-lane part of an interior address is an
+// Work unit: one wave64 per 64-node x-segment ("tile") of the box, which is padded to at least
+// one extra column in x (to a multiple of 64) and one extra row in y, so that no out-of-box
+// neighbour position aliases a node.  A tile descriptor carries three 64-bit lane masks (lattice
+// node, next-to-wall, boundary) and the first side-table slot of each kind, so a lane finds its
+// slot with mbcnt and no per-lane lookup.  Each pull is one contiguous 512-byte (FP64) / 128-byte
+// (16-bit) run of a direction plane shifted by the stencil offset; every plane has a guard before
+// and after so that all pull addresses are in bounds and the per-lane part of an address is an
 // unsigned 32-bit byte offset from a scalar plane base.
 
 #include <hip/hip_runtime.h>
@@ -89,28 +90,21 @@ __device__ __forceinline__ void load_all(const StepParams& p, uint32_t nb, typen
     }
 }
 
-// Lanes of a tile with boundary nodes: each pull reads the lattice, or the stale table where the
-// link has no upstream node (the neighbour in direction opp(D) is missing).
+// Stale populations live in the lattice itself: a link of node x with no upstream node pulls from
+// slot x - c_D of plane D, a non-node slot (solid, or padding / guard outside the box) that only x
+// ever addresses in that plane, so x parks its post-closure value there for the next step.
 template <int Q, class ST, int D = 0>
-__device__ __forceinline__ void load_select(const StepParams& p, uint32_t nb, uint32_t nmiss,
-                                            int32_t bslot, typename ST::S (&v)[Q]) {
-    if constexpr (D < Q) {
-        using S = typename ST::S;
-        const bool miss = (nmiss >> Lat<Q>::opp[D]) & 1u;
-        const S* a = miss ? static_cast<const S*>(p.P) + D * p.pst + bslot : pull_addr<Q, S, D>(p, nb);
-        v[D] = *a;
-        load_select<Q, ST, D + 1>(p, nb, nmiss, bslot, v);
-    }
-}
-
-template <int Q, class ST, int D = 0>
-__device__ __forceinline__ void save_stale(const StepParams& p, uint32_t nmiss, int32_t bslot,
+__device__ __forceinline__ void save_stale(const StepParams& p, uint32_t nb, uint32_t nmiss,
                                            const typename ST::R (&f)[Q]) {
     if constexpr (D < Q) {
         using S = typename ST::S;
-        if ((nmiss >> Lat<Q>::opp[D]) & 1u)
-            static_cast<S*>(p.P)[D * p.pst + bslot] = ST::store(f[D], Lat<Q>::w[D]);
-        save_stale<Q, ST, D + 1>(p, nmiss, bslot, f);
+        if ((nmiss >> Lat<Q>::opp[D]) & 1u) {
+            constexpr int64_t cx = Lat<Q>::cx[D], cy = Lat<Q>::cy[D], cz = Lat<Q>::cz[D];
+            const int64_t offb = (cx + cy * p.sy + cz * p.sz) * (int64_t)sizeof(S);
+            char* plane = static_cast<char*>(p.B) + D * p.ns * (int64_t)sizeof(S);
+            *reinterpret_cast<S*>(plane + (uint32_t)(nb - (uint32_t)offb)) = ST::store(f[D], Lat<Q>::w[D]);
+        }
+        save_stale<Q, ST, D + 1>(p, nb, nmiss, f);
     }
 }
 
@@ -164,8 +158,7 @@ __device__ __forceinline__ bool node_update(const StepParams& p, double (&red)[4
     R f[Q];
     if (!init) {
         S v[Q];
-        if (T.bnd == 0) load_all<Q, ST>(p, nb, v);
-        else load_select<Q, ST>(p, nb, nmiss, bslot, v);
+        load_all<Q, ST>(p, nb, v);
 #pragma unroll
         for (int d = 0; d < Q; d++) f[d] = ST::load(v[d], Lat<Q>::w[d]);
         if constexpr (ST::kGate) {
@@ -195,7 +188,7 @@ __device__ __forceinline__ bool node_update(const StepParams& p, double (&red)[4
         init_populations<Q>(p, n, f);
         if (bnd) cl = p.bclose[bslot];
     }
-    if (bnd) save_stale<Q, ST>(p, nmiss, bslot, f);
+    if (bnd) save_stale<Q, ST>(p, nb, nmiss, f);
 
     R rho, u[3];
     moments(f, rho, u);

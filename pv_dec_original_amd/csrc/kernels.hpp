@@ -47,8 +47,6 @@ struct StepParams {
     const uint8_t* bclose;  // enum BClose
     const double* bprof;    // inlet velocity profile
     const double* brho;     // prescribed outlet density
-    void* P;                // stale populations [Q][pst] (storage type, shifted like the lattice)
-    int64_t pst;
     // next-to-wall table
     void* M;                // previous (rho, ux, uy, uz) [4][mst] (arithmetic type)
     int64_t mst;
@@ -84,11 +82,12 @@ struct VoxParams {
 hipError_t launch_step(int q, int storage, const StepParams& p, hipStream_t s);
 hipError_t launch_reduce_final(const double* partials, int nparts, double* out4, hipStream_t s);
 hipError_t launch_voxelize(const VoxParams& p, hipStream_t s);
-// boundary flags: per padded node, 1 if fluid and a Q neighbour is missing
-hipError_t launch_boundary(int q, const uint8_t* fluid_pad, int64_t nxp, int64_t ny, int64_t nz,
-                           int64_t nx, uint8_t* bflag, hipStream_t s);
+// boundary flags: per padded node, 1 if fluid and a Q neighbour is missing (padded box nxp x nyp
+// x nz holding the nx x ny x nz grid)
+hipError_t launch_boundary(int q, const uint8_t* fluid_pad, int64_t nxp, int64_t nyp, int64_t nx,
+                           int64_t ny, int64_t nz, uint8_t* bflag, hipStream_t s);
 // next-to-wall marking: FLUID nodes with a WALL Q-neighbour become NTW
-hipError_t launch_ntw(int q, uint8_t* kind, int64_t nxp, int64_t ny, int64_t nz, hipStream_t s);
+hipError_t launch_ntw(int q, uint8_t* kind, int64_t nxp, int64_t nyp, int64_t nz, hipStream_t s);
 // per-triangle unit normal, area, centroid (Subdomain.cpp:244-267)
 hipError_t launch_tri_prep(const double* tri, int64_t ntri, double* prep /*[8][ntri]*/, hipStream_t s);
 // area/d^2-weighted normal of each listed node (Subdomain.cpp:301-369)

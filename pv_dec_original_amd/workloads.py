@@ -117,6 +117,32 @@ def aneurysm_case(seed=7, q=19, box=(1024, 512, 512), tau=0.6):
     return Case("aneurysm", geo, xfaces(geo), tau, (0.0, 0.0), 40.0)
 
 
+def fluid_box_case(q=19, box=(512, 256, 256), tau=0.6):
+    """An all-fluid box (every grid point a node): the step kernel's ceiling, with inlet and outlet
+    on the x faces and walls on the other four.  The triangles are the box's own 12 facets."""
+    nx, ny, nz = box
+    c = np.array([[0, 0, 0], [nx - 1, 0, 0], [nx - 1, ny - 1, 0], [0, ny - 1, 0],
+                  [0, 0, nz - 1], [nx - 1, 0, nz - 1], [nx - 1, ny - 1, nz - 1], [0, ny - 1, nz - 1]],
+                 dtype=np.float64)
+    quads = [(0, 3, 2, 1), (4, 5, 6, 7), (0, 1, 5, 4), (2, 3, 7, 6), (0, 4, 7, 3), (1, 2, 6, 5)]
+    tri = []
+    for a, b, cc, d in quads:
+        for v1, v2, v3 in ((a, b, cc), (a, cc, d)):
+            n = np.cross(c[v2] - c[v1], c[v3] - c[v1])
+            tri.append(np.concatenate([c[v1], c[v2], c[v3], n / np.linalg.norm(n)]))
+    geo = Geometry.__new__(Geometry)
+    geo.m_velSet = LatticeModel(q)
+    geo.spacing = 1.0
+    geo.raydirection = "z"
+    geo.globalTriangles = np.array(tri)
+    geo.boundingbox = np.array([0, nx - 1, 0, ny - 1, 0, nz - 1], dtype=np.float64)
+    geo.fluid = np.ones((nz, ny, nx), dtype=np.uint8)
+    geo.origin = np.zeros(3)
+    geo.nx, geo.ny, geo.nz = nx, ny, nz
+    geo.stlFileName = None
+    return Case("box", geo, xfaces(geo), tau, (ny / 2.0, nz / 2.0), min(ny, nz) / 2.0)
+
+
 def inlet_disk_profile(geo, subdomain, u_mean):
     """Parabolic profile on an inlet of arbitrary shape: per connected inlet set, fit the
     centre and equivalent radius from the node coordinates and return u(r) with mean u_mean."""
