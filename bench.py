@@ -40,6 +40,20 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def measured_traffic(workload):
+    """HBM bytes per launch of the step kernel from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary of the same workload (calibrated against a copy kernel), or None."""
+    name = {"ybif-D3Q27-fp64-512x256x256": "k_step_ybif512_q27_fp64.json",
+            "ybif-D3Q19-fp16-512x256x256": "k_step_ybif512_q19_fp16.json"}.get(workload)
+    if not name:
+        return None
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    with open(path) as fp:
+        return round(json.load(fp)["traffic_bytes"])
+
+
 def inlet_scales(it0, n, period, mean, amp):
     t = np.arange(it0, it0 + n, dtype=np.float64)
     return mean + amp * np.sin(2.0 * np.pi * t / period)
@@ -157,6 +171,7 @@ def main():
                          "normals skipped (timing only)"}
         log(f"cpu baseline {v:.2f} MLUPS ({n} steps)")
 
+    workload = f"{args.workload}-D3Q{q}-{args.storage}-{box[0]}x{box[1]}x{box[2]}"
     out = {
         "metric": "MLUPS + % HBM roofline",
         "value": round(mlups, 2),
@@ -171,7 +186,7 @@ def main():
         "dtype": dtype,
         "data": "synthetic (seeded generated mesh, voxelised on the GPU)",
         "config": {
-            "workload": f"{args.workload}-D3Q{q}-{args.storage}-{box[0]}x{box[1]}x{box[2]}",
+            "workload": workload,
             "lattice": q, "storage": args.storage, "box": list(box), "fluid_nodes": nfl,
             "fluid_fraction": round(nfl / nbox, 4), "tau": tau,
             "inlet": "0.03+0.02*sin(2*pi*t/4000) x parabolic(mean 1)", "wss_every": args.wss_every,
@@ -180,7 +195,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(workload),
             "bytes_per_flup": bytes_per_flup,
             "kernel": f"k_step<{q},{args.storage}>", "launch_avg_us": round(per_launch_s * 1e6, 2),
             "copy_kernel_GBs": None if copy_gbs is None else round(copy_gbs, 1),

@@ -1,0 +1,190 @@
+/*
+ * pvlbm.hpp — C++ facade with the reference solver's class names over the C ABI of pvlbm.h.
+ *
+ * The reference builds its solver as  LatticeModel -> Geometry -> SubDomain -> Simulation
+ * (LBM.cpp:69-72).  These classes keep those names, constructor arguments and call order; the
+ * state lives on the GPU behind a pvlbm_handle.  Errors throw pvlbm::Error (the reference prints
+ * and continues, Simulation.cpp:298-300; a device solver cannot continue after a failed call).
+ * Header-only; link with libpvlbm.so.
+ */
+#ifndef PVLBM_HPP
+#define PVLBM_HPP
+
+#include <cmath>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "pvlbm.h"
+
+namespace pvlbm {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+inline void check(int rc, pvlbm_handle h = nullptr) {
+    if (rc != PVLBM_OK) throw Error(rc, pvlbm_last_error(h));
+}
+
+/* LatticeModel (LatticeModel.h:9): the reference picks D3Q19 or D3Q27 at compile time
+ * (Macro.h:6); here it is a constructor argument. */
+class LatticeModel {
+public:
+    explicit LatticeModel(int q = 19) : q_(q) {
+        if (q != 19 && q != 27) throw Error(PVLBM_EINVAL, "lattice must be 19 or 27");
+    }
+    int Get_nDirections() const { return q_; }
+    int Get_nDimensions() const { return 3; }
+
+private:
+    int q_;
+};
+
+/* Geometry(FileName, velSet, spacing, raydirection) (ComplexGeometry.h:91): reads the ASCII STL,
+ * takes the reference bbox and grid and labels the points by +z ray parity on the GPU.  A second
+ * constructor takes triangles already in memory and an optional explicit grid. */
+class Geometry {
+public:
+    Geometry(const std::string& FileName, const LatticeModel& velSet, double spacing, char raydirection = 'z')
+        : velSet_(velSet), spacing(spacing), raydirection(raydirection) {
+        int64_t n = 0;
+        check(pvlbm_read_stl_ascii(FileName.c_str(), nullptr, 0, &n));
+        if (n == 0) throw Error(PVLBM_EIO, "no facets read from " + FileName);
+        triangles.resize((size_t)(12 * n));
+        check(pvlbm_read_stl_ascii(FileName.c_str(), triangles.data(), n, &n));
+        label(nullptr);
+    }
+    Geometry(std::vector<double> tri, const LatticeModel& velSet, double spacing,
+             const pvlbm_grid* grid = nullptr)
+        : velSet_(velSet), spacing(spacing), raydirection('z'), triangles(std::move(tri)) {
+        label(grid);
+    }
+
+    const LatticeModel& velSet() const { return velSet_; }
+    int64_t ntri() const { return (int64_t)(triangles.size() / 12); }
+    double coord(int axis, int64_t idx) const { return grid.origin[axis] + (double)idx * spacing; }
+
+    LatticeModel velSet_;
+    double spacing;
+    char raydirection;  // stored; labelling always casts +z rays (ComplexGeometry.cpp:994)
+    std::vector<double> triangles;
+    double boundingbox[6] = {0, 0, 0, 0, 0, 0};
+    pvlbm_grid grid{};
+    std::vector<uint8_t> fluid;  // nx*ny*nz, x fastest
+
+private:
+    void label(const pvlbm_grid* g) {
+        check(pvlbm_bbox(triangles.data(), ntri(), boundingbox));
+        check(pvlbm_voxelize(triangles.data(), ntri(), spacing, g, &grid, nullptr));
+        fluid.resize((size_t)grid.dims[0] * grid.dims[1] * grid.dims[2]);
+        check(pvlbm_voxelize(triangles.data(), ntri(), spacing, g, &grid, fluid.data()));
+    }
+};
+
+/* SubDomain(Geometry&) (Subdomain.h:66) with the boundary faces that the reference hard-codes
+ * (ComplexGeometry.cpp:1378-1408) passed in; storage selects FP64 or the shifted 16-bit forms.
+ * tau and del_t take the reference defaults until a Simulation sets them. */
+class SubDomain {
+public:
+    SubDomain(const Geometry& geo, const std::vector<pvlbm_face>& faces, int storage = PVLBM_FP64,
+              double tau = 1.0, double del_t = 8e-6)
+        : geo_(&geo) {
+        check(pvlbm_create(geo.fluid.data(), &geo.grid, geo.spacing, geo.velSet().Get_nDirections(),
+                           storage, faces.empty() ? nullptr : faces.data(), (int)faces.size(),
+                           geo.triangles.data(), geo.ntri(), tau, del_t, &h_));
+    }
+    ~SubDomain() {
+        if (h_) pvlbm_destroy(h_);
+    }
+    SubDomain(const SubDomain&) = delete;
+    SubDomain& operator=(const SubDomain&) = delete;
+
+    std::vector<int64_t> list(int which) const {
+        int64_t n = 0;
+        check(pvlbm_count(h_, which, &n), h_);
+        std::vector<int64_t> v((size_t)n);
+        if (n) check(pvlbm_list(h_, which, v.data()), h_);
+        return v;
+    }
+    int64_t count(int which) const {
+        int64_t n = 0;
+        check(pvlbm_count(h_, which, &n), h_);
+        return n;
+    }
+    std::vector<int64_t> InletBoundaryNode_() const { return list(PVLBM_LIST_INLET); }
+    std::vector<int64_t> WallBoundaryNode_() const { return list(PVLBM_LIST_WALL); }
+    std::vector<int64_t> nextToTheWall() const { return list(PVLBM_LIST_NTW); }
+    const Geometry& Geometry_() const { return *geo_; }
+    pvlbm_handle handle() const { return h_; }
+
+private:
+    const Geometry* geo_;
+    pvlbm_handle h_ = nullptr;
+};
+
+/* Simulation(SubDomain&, tau, pressure, del_t) (Simulation.h:26).  The reference runs its whole
+ * loop inside the constructor (Simulation.cpp:13); here Iterate() does, and step() advances any
+ * number of iterations of the loop body (Simulation.cpp:577-586). */
+class Simulation {
+public:
+    Simulation(SubDomain& sd, double tau, double pressure, double del_t)
+        : sd_(sd), h_(sd.handle()), pressure_(pressure), m_dt(del_t) {
+        SetRelaxation(tau);
+        check(pvlbm_set_dt(h_, del_t), h_);
+    }
+    void SetRelaxation(double tau) {
+        m_relaxation = tau;
+        check(pvlbm_set_tau(h_, tau), h_);
+    }
+    void setInletProfile(const std::vector<double>& u) { check(pvlbm_set_inlet_profile(h_, u.data()), h_); }
+    void setWssEvery(int every) { check(pvlbm_set_wss_every(h_, every), h_); }
+    void step(int n, const double* inlet_scale = nullptr) { check(pvlbm_step(h_, n, inlet_scale), h_); }
+    static double getInletVelocity(double t, double period) { return pvlbm_pulsatile_velocity(t, period); }
+
+    /* Simulation::Iterate (Simulation.cpp:569-601): while total_runtime <= t_end, one iteration
+     * with the pulsatile inlet at total_runtime, total_runtime += dt.  every_iter, when set, is
+     * called after each iteration whose index is a multiple of output_every. */
+    template <class F>
+    void Iterate(double t_end, double period, int output_every, F every_iter) {
+        total_runtime = 0.0;
+        int64_t it = 0;
+        std::vector<double> sc;
+        while (total_runtime <= t_end) {
+            sc.clear();
+            int64_t n = 0;
+            double t = total_runtime;
+            while (t <= t_end && n < output_every - (it % output_every)) {
+                sc.push_back(getInletVelocity(t, period));
+                t += m_dt;
+                n++;
+            }
+            step((int)n, sc.data());
+            it += n;
+            total_runtime = t;
+            if (it % output_every == 0 || total_runtime > t_end) every_iter(it - 1);
+        }
+    }
+
+    /* x, y, z, WSS, OSI, TAWSS per next-to-wall node (OutputFile.cpp:884-894) */
+    std::vector<double> surfaceAttributes() const {
+        std::vector<double> out((size_t)(6 * sd_.count(PVLBM_LIST_NTW)));
+        if (!out.empty()) check(pvlbm_get_wss(h_, out.data()), h_);
+        return out;
+    }
+
+    double total_runtime = 0.0;
+
+private:
+    SubDomain& sd_;
+    pvlbm_handle h_;
+    double pressure_;  // stored; the outlets use their face density (Simulation.cpp:630)
+    double m_dt;
+    double m_relaxation = 1.0;
+};
+
+}  // namespace pvlbm
+
+#endif

@@ -2,6 +2,7 @@
 
 libpvlbm.so  — the gfx950 HIP kernels and the C ABI of include/pvlbm.h (hipcc)
 libpvgen.so  — synthetic vessel mesh generators of include/pvgen.h (g++)
+lbm_run      — command-line driver over the C++ facade include/pvlbm.hpp (g++)
 
 Every HIP translation unit is compiled with -ffp-contract=off so that the FP64 path keeps the
 reference's operation order without fused multiply-adds, and with packed FP32 instructions
@@ -48,6 +49,12 @@ def build(verbose=False):
     _run(["g++", "-O2", "-fopenmp", "-fPIC", "-shared", "-std=c++17", "-Wall",
           os.path.join(CSRC, "geomgen.cpp"), "-o", gen + ".tmp"], verbose)
     os.replace(gen + ".tmp", gen)
+    # the command-line driver over the C++ facade (include/pvlbm.hpp)
+    exe = os.path.join(HERE, "lbm_run")
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
+          os.path.join(CSRC, "lbm_run.cpp"), "-L", HERE, "-lpvlbm", "-Wl,-rpath,$ORIGIN",
+          "-o", exe + ".tmp"], verbose)
+    os.replace(exe + ".tmp", exe)
 
 
 def build_oracle(verbose=False):
