@@ -1,4 +1,4 @@
-"""ctypes bindings of the in-tree native libraries (include/pvlbm.h, include/pvgen.h).
+"""ctypes bindings of the in-tree native libraries (include/pvlbm.h, include/pvnet.h, include/pvgen.h).
 
 The simulation path has no fallback: if libpvlbm.so is missing or a call fails, an exception is
 raised.  Build with ``python -m pv_dec_original_amd.build`` or ``__graft_entry__.build()``.
@@ -36,8 +36,31 @@ class NativeError(RuntimeError):
     pass
 
 
+class NetShape(C.Structure):
+    """pvnet_shape: a batch of nb boxes of d x h x w voxels."""
+    _fields_ = [("nb", C.c_int32), ("d", C.c_int32), ("h", C.c_int32), ("w", C.c_int32)]
+
+
+class NetConv(C.Structure):
+    """pvnet_conv (forward / input-gradient convolution)."""
+    _fields_ = [("s", NetShape), ("ntaps", C.c_int32), ("cin", C.c_int32), ("cout", C.c_int32),
+                ("kpad", C.c_int32), ("x", C.c_void_p), ("ldx", C.c_int32), ("flags", C.c_int32),
+                ("w", C.c_void_p), ("bias", C.c_void_p), ("y", C.c_void_p), ("ldy", C.c_int32),
+                ("ldr", C.c_int32), ("relu_of", C.c_void_p)]
+
+
+class NetWgrad(C.Structure):
+    """pvnet_wgrad (weight and bias gradient)."""
+    _fields_ = [("s", NetShape), ("ntaps", C.c_int32), ("cin", C.c_int32), ("cout", C.c_int32),
+                ("x", C.c_void_p), ("ldx", C.c_int32), ("lddy", C.c_int32), ("dy", C.c_void_p),
+                ("ws", C.c_void_p), ("dw", C.c_void_p), ("db", C.c_void_p)]
+
+
+NET_RELU, NET_F32OUT = 1, 2
+
 _lib = None
 _gen = None
+_net = None
 
 # (name, restype, argtypes) of every entry point of include/pvlbm.h
 PVLBM_API = [
@@ -68,6 +91,27 @@ PVLBM_API = [
     ("pvlbm_pulsatile_velocity", C.c_double, [C.c_double, C.c_double]),
     ("pvlbm_destroy", C.c_int, [C.c_void_p]),
     ("pvlbm_copy_bench", C.c_int, [C.c_int64, C.c_int, _d]),
+]
+
+_vp = C.c_void_p
+_sp = C.POINTER(NetShape)
+PVNET_API = [
+    ("pvnet_conv_fwd", C.c_int, [C.POINTER(NetConv), _vp]),
+    ("pvnet_wgrad_ws_floats", C.c_int64, [_sp, C.c_int32, C.c_int32, C.c_int32]),
+    ("pvnet_conv_wgrad", C.c_int, [C.POINTER(NetWgrad), _vp]),
+    ("pvnet_pack_weights", C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32, _vp, _vp, _vp]),
+    ("pvnet_pool2_fwd", C.c_int, [_sp, C.c_int32, _vp, C.c_int32, _vp, C.c_int32, _vp]),
+    ("pvnet_pool2_bwd", C.c_int, [_sp, C.c_int32, _vp, C.c_int32, _vp, C.c_int32, C.c_int32, _vp]),
+    ("pvnet_up2_fwd", C.c_int, [_sp, C.c_int32, _vp, C.c_int32, _vp, C.c_int32, _vp]),
+    ("pvnet_up2_bwd", C.c_int, [_sp, C.c_int32, _vp, C.c_int32, _vp, C.c_int32, _vp]),
+    ("pvnet_relu_bwd", C.c_int, [C.c_int64, C.c_int32, _vp, C.c_int32, _vp, C.c_int32, _vp]),
+    ("pvnet_features", C.c_int, [_sp, _vp, _vp, _vp, _vp]),
+    ("pvnet_loss_ws_doubles", C.c_int64, [C.c_int64]),
+    ("pvnet_mse_loss", C.c_int, [C.c_int64, C.c_int32, _vp, C.c_int32, _vp, _vp, _vp, _vp, C.c_int32,
+                                 _vp, _vp, _vp]),
+    ("pvnet_adam", C.c_int, [C.c_int64, _vp, _vp, _vp, _vp, C.c_float, C.c_float, C.c_float,
+                             C.c_float, C.c_int32, _vp]),
+    ("pvnet_last_error", C.c_char_p, []),
 ]
 
 PVGEN_API = [
@@ -105,6 +149,23 @@ def lib():
             raise NativeError(f"{path} is missing: build it with __graft_entry__.build()")
         _lib = _bind(C.CDLL(path), PVLBM_API)
     return _lib
+
+
+def net():
+    """The loaded libpvnet.so (warm-start network kernels); raises if it has not been built."""
+    global _net
+    if _net is None:
+        path = os.path.join(HERE, "libpvnet.so")
+        if not os.path.exists(path):
+            raise NativeError(f"{path} is missing: build it with __graft_entry__.build()")
+        _net = _bind(C.CDLL(path), PVNET_API)
+    return _net
+
+
+def net_check(rc):
+    if rc != PVLBM_OK:
+        msg = net().pvnet_last_error()
+        raise NativeError(f"pvnet error {rc}: {msg.decode() if msg else ''}")
 
 
 def gen():

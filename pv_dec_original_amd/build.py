@@ -1,6 +1,7 @@
 """Builds the native libraries in-tree.
 
 libpvlbm.so  — the gfx950 HIP kernels and the C ABI of include/pvlbm.h (hipcc)
+libpvnet.so  — the warm-start network's gfx950 kernels and the C ABI of include/pvnet.h (hipcc)
 libpvgen.so  — synthetic vessel mesh generators of include/pvgen.h (g++)
 lbm_run      — command-line driver over the C++ facade include/pvlbm.hpp (g++)
 
@@ -25,6 +26,7 @@ HIP_FLAGS = [
 
 LIBS = {
     "libpvlbm.so": ["capi.cpp", "k_step.hip", "k_setup.hip"],
+    "libpvnet.so": ["k_net.hip"],
 }
 
 

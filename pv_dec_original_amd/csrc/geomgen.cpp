@@ -316,27 +316,52 @@ extern "C" int pvgen_vessel(uint64_t seed, double n, double rmin, double rmax, i
     if (!out || spacing <= 0 || rmax < rmin) return 1;
     Rng rng(seed * 0x2545F4914F6CDD1Dull + 12345);
     const double m = rmax + 3.0;  // keep every tube inside the box in y and z
+    // Inlet and outlets are straight legs along x (flow extensions) crossing the planes x = 0 and
+    // x = n: the Zou-He closures of the x faces take the wall to run along x at the boundary
+    // plane, and an oblique vessel end leaves rim nodes whose closure is unstable.
+    const double x_in = 0.12 * n, x_out = 0.88 * n, leg0 = -0.2 * n, leg1 = 1.2 * n;
     std::vector<Seg> segs;
     int ncp = rng.irange(4, 6);
     std::vector<std::array<double, 4>> main;
     for (int i = 0; i < ncp; i++) {
-        double x = -0.15 * n + (1.3 * n) * i / (ncp - 1);
+        double x = x_in + (x_out - x_in) * i / (ncp - 1);
         main.push_back({x, rng.uni(m, n - m), rng.uni(m, n - m), rng.uni(rmin, rmax)});
     }
+    auto straight = [&](const std::array<double, 4>& p, double x) {
+        Seg s = {{p[0], p[1], p[2]}, {x, p[1], p[2]}, p[3], p[3]};
+        segs.push_back(s);
+    };
+    straight(main.front(), leg0);
+    straight(main.back(), leg1);
+    size_t first = segs.size();
     spline_segments(main, 24, segs);
-    size_t nmain = segs.size();
+    size_t nmain = segs.size() - first;
+    std::vector<std::array<double, 3>> exits = {{main.back()[1], main.back()[2], main.back()[3]}};
     for (int b = 0; b < std::max(1, std::min(2, nbif)); b++) {
-        // branch from a point of the main line towards x > n
-        size_t at = (size_t)rng.uni(0.2, 0.55) * 0 + (size_t)(nmain * rng.uni(0.2, 0.55));
-        const Seg& s0 = segs[std::min(at, nmain - 1)];
+        // branch from a point of the main line to an exit of its own at x = n
+        size_t at = first + (size_t)(nmain * rng.uni(0.2, 0.55));
+        const Seg& s0 = segs[std::min(at, segs.size() - 1)];
         double r0 = std::max(rmin, 0.75 * s0.ra);
         std::vector<std::array<double, 4>> br;
         br.push_back({s0.a[0], s0.a[1], s0.a[2], r0});
         int nb = rng.irange(2, 3);
         for (int i = 1; i <= nb; i++) {
-            double x = s0.a[0] + (1.15 * n - s0.a[0]) * i / nb;
+            double x = s0.a[0] + (x_out - s0.a[0]) * i / nb;
             br.push_back({x, rng.uni(m, n - m), rng.uni(m, n - m), rng.uni(rmin, std::max(rmin, r0))});
         }
+        // keep the exit apart from the others where the box allows it (a bounded number of draws)
+        for (int tries = 0; tries < 32; tries++) {
+            bool apart = true;
+            for (const auto& e : exits) {
+                double dy = br.back()[1] - e[0], dz = br.back()[2] - e[1];
+                apart = apart && std::sqrt(dy * dy + dz * dz) > br.back()[3] + e[2] + 4.0;
+            }
+            if (apart) break;
+            br.back()[1] = rng.uni(m, n - m);
+            br.back()[2] = rng.uni(m, n - m);
+        }
+        exits.push_back({br.back()[1], br.back()[2], br.back()[3]});
+        straight(br.back(), leg1);
         spline_segments(br, 24, segs);
     }
     Grid G = make_grid(lo, hi, spacing);

@@ -1,0 +1,704 @@
+// k_net.hip — kernels of the warm-start network and the C ABI of include/pvnet.h (gfx950).
+//
+// The convolutions are implicit GEMMs on the CDNA4 matrix cores (v_mfma_f32_16x16x32_bf16):
+//   forward / input gradient   C[m][n] = sum_k A[m][k] B[k][n],  m = voxel, n = output channel,
+//                              k = tap * cin + ci, A gathered from the channels-last activations
+//                              with zero padding, B the packed bf16 weights [n][k].
+//   weight gradient            C[n][k] = sum_m dy[m][n] x[m + off(tap(k))][ci(k)], split over the
+//                              voxels into FP32 partials that a second kernel sums in fixed order.
+// Tiles are staged through LDS in rows of 32 k-values (one MFMA depth); the next tile's global
+// loads are issued before the current tile's MFMAs so that their latency overlaps the math.
+//
+// MFMA 16x16x32 bf16 lane layout (64 lanes, g = lane / 16, c = lane % 16):
+//   A operand: A[c][8g + j], j = 0..7      B operand: B[8g + j][c]
+//   D:         D[4g + e][c], e = 0..3
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <string>
+
+#include "../../include/pvnet.h"
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define NET_HIP(x)                                                                      \
+    do {                                                                                \
+        hipError_t e_ = (x);                                                            \
+        if (e_ != hipSuccess) return fail(2, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define NET_LAUNCH()                                                                    \
+    do {                                                                                \
+        hipError_t e_ = hipGetLastError();                                              \
+        if (e_ != hipSuccess) return fail(2, std::string("launch: ") + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int BK = 32;  // k-values per MFMA step
+constexpr int LP = 40;  // LDS row pitch in bf16 (32 + 8 padding: 80-byte rows)
+
+__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+// round to nearest even; NaN stays NaN
+__device__ __forceinline__ uint16_t f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+// bf16 bit pattern strictly greater than zero (NaN is not)
+__device__ __forceinline__ bool bf_pos(uint16_t b) { return (b & 0x8000u) == 0 && b != 0 && b <= 0x7f80u; }
+
+__device__ __forceinline__ void tap_offset(int ntaps, int tap, int& dz, int& dy, int& dx) {
+    if (ntaps == 27) {
+        dz = tap / 9 - 1;
+        dy = (tap / 3) % 3 - 1;
+        dx = tap % 3 - 1;
+    } else {
+        dz = dy = dx = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// forward / input-gradient convolution: block tile 128 voxels x BN channels, 4 waves of 32 x BN
+
+template <int BN>
+__global__ __launch_bounds__(256) void k_conv_fwd(pvnet_conv a, int M, int K) {
+    constexpr int BM = 128;
+    __shared__ __attribute__((aligned(16))) uint16_t As[BM * LP];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[BN * LP];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int g = lane >> 4, c16 = lane & 15;
+    const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+    const int d = a.s.d, h = a.s.h, w = a.s.w;
+
+    // load roles: A rows r and r + 64, 8-wide k chunk q; B row bn (if < BN), same chunk
+    const int q = t & 3, r = t >> 2, bn = t >> 2;
+    int zz[2], yy[2], xx[2];
+    bool ok[2];
+    for (int i = 0; i < 2; i++) {
+        int m = m0 + r + 64 * i;
+        ok[i] = m < M;
+        int mm = ok[i] ? m : 0;
+        xx[i] = mm % w;
+        int tq = mm / w;
+        yy[i] = tq % h;
+        zz[i] = (tq / h) % d;
+    }
+    const bool brow = bn < BN && n0 + bn < a.cout;
+    const int nsteps = a.kpad / BK;
+    uint4 ra[2], rb;
+
+    auto load = [&](int s) {
+        const int k = s * BK + q * 8;
+        const bool kin = k < K;
+        const int tap = kin ? k / a.cin : 0;
+        const int ci = k - tap * a.cin;
+        int dz, dy, dx;
+        tap_offset(a.ntaps, tap, dz, dy, dx);
+        const int64_t off = ((int64_t)dz * h + dy) * w + dx;
+        for (int i = 0; i < 2; i++) {
+            const int z2 = zz[i] + dz, y2 = yy[i] + dy, x2 = xx[i] + dx;
+            const bool in = ok[i] && kin && (unsigned)z2 < (unsigned)d && (unsigned)y2 < (unsigned)h &&
+                            (unsigned)x2 < (unsigned)w;
+            ra[i] = make_uint4(0, 0, 0, 0);
+            if (in) ra[i] = *(const uint4*)(a.x + ((int64_t)(m0 + r + 64 * i) + off) * a.ldx + ci);
+        }
+        rb = make_uint4(0, 0, 0, 0);
+        if (brow) rb = *(const uint4*)(a.w + (int64_t)(n0 + bn) * a.kpad + s * BK + q * 8);
+    };
+
+    f32x4 acc[2][BN / 16];
+    for (int i = 0; i < 2; i++)
+        for (int j = 0; j < BN / 16; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    load(0);
+    for (int s = 0; s < nsteps; s++) {
+        for (int i = 0; i < 2; i++) *(uint4*)(As + (r + 64 * i) * LP + q * 8) = ra[i];
+        if (bn < BN) *(uint4*)(Bs + bn * LP + q * 8) = rb;
+        __syncthreads();
+        if (s + 1 < nsteps) load(s + 1);
+        bf16x8 af[2], bfr[BN / 16];
+        for (int i = 0; i < 2; i++) af[i] = *(const bf16x8*)(As + (wv * 32 + i * 16 + c16) * LP + g * 8);
+        for (int j = 0; j < BN / 16; j++) bfr[j] = *(const bf16x8*)(Bs + (j * 16 + c16) * LP + g * 8);
+        for (int i = 0; i < 2; i++)
+            for (int j = 0; j < BN / 16; j++)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+        __syncthreads();
+    }
+
+    const bool relu = a.flags & PVNET_RELU, f32out = a.flags & PVNET_F32OUT;
+    for (int i = 0; i < 2; i++)
+        for (int j = 0; j < BN / 16; j++) {
+            const int n = n0 + j * 16 + c16;
+            if (n >= a.cout) continue;
+            const float b = a.bias ? a.bias[n] : 0.f;
+            for (int e = 0; e < 4; e++) {
+                const int m = m0 + wv * 32 + i * 16 + g * 4 + e;
+                if (m >= M) continue;
+                float v = acc[i][j][e] + b;
+                if (relu) v = v < 0.f ? 0.f : v;
+                if (a.relu_of && !bf_pos(a.relu_of[(int64_t)m * a.ldr + n])) v = 0.f;
+                if (f32out)
+                    ((float*)a.y)[(int64_t)m * a.ldy + n] = v;
+                else
+                    ((uint16_t*)a.y)[(int64_t)m * a.ldy + n] = f2bf(v);
+            }
+        }
+}
+
+// ---------------------------------------------------------------------------------------------
+// weight gradient: block tile BN output channels x 64 k-values, reduction over its voxel chunk
+
+template <int BN>
+__global__ __launch_bounds__(256) void k_conv_wgrad(pvnet_wgrad a, int M, int K, int mchunk) {
+    __shared__ __attribute__((aligned(16))) uint16_t As[BN * LP];  // dy^T  [n][m]
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[64 * LP];  // x^T   [k][m]
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int g = lane >> 4, c16 = lane & 15;
+    const int k0 = blockIdx.x * 64, n0 = blockIdx.y * BN, split = blockIdx.z;
+    const int mb = split * mchunk, me = min(M, mb + mchunk);
+    const int d = a.s.d, h = a.s.h, w = a.s.w;
+
+    const int mm = t >> 3, c8 = (t & 7) * 8;
+    const int k = k0 + c8;
+    const bool kin = k < K;
+    const int tap = kin ? k / a.cin : 0;
+    const int ci = k - tap * a.cin;
+    int dz, dy, dx;
+    tap_offset(a.ntaps, tap, dz, dy, dx);
+    const int64_t off = ((int64_t)dz * h + dy) * w + dx;
+    const bool drow = c8 < BN && n0 + c8 < a.cout;
+
+    f32x4 acc[BN / 16];
+    for (int j = 0; j < BN / 16; j++) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    uint4 vd, vx;
+    auto load = [&](int mbase) {
+        const int m = mbase + mm;
+        vd = make_uint4(0, 0, 0, 0);
+        if (drow) vd = *(const uint4*)(a.dy + (int64_t)m * a.lddy + n0 + c8);
+        const int x = m % w, tq = m / w, y = tq % h, z = (tq / h) % d;
+        const int z2 = z + dz, y2 = y + dy, x2 = x + dx;
+        const bool in = kin && (unsigned)z2 < (unsigned)d && (unsigned)y2 < (unsigned)h && (unsigned)x2 < (unsigned)w;
+        vx = make_uint4(0, 0, 0, 0);
+        if (in) vx = *(const uint4*)(a.x + ((int64_t)m + off) * a.ldx + ci);
+    };
+
+    if (mb < me) load(mb);
+    for (int mbase = mb; mbase < me; mbase += 32) {
+        const uint16_t* pd = (const uint16_t*)&vd;
+        const uint16_t* px = (const uint16_t*)&vx;
+        if (c8 < BN)
+            for (int j = 0; j < 8; j++) As[(c8 + j) * LP + mm] = pd[j];
+        for (int j = 0; j < 8; j++) Bs[(c8 + j) * LP + mm] = px[j];
+        __syncthreads();
+        if (mbase + 32 < me) load(mbase + 32);
+        const bf16x8 bx = *(const bf16x8*)(Bs + (wv * 16 + c16) * LP + g * 8);
+        for (int j = 0; j < BN / 16; j++) {
+            const bf16x8 ad = *(const bf16x8*)(As + (j * 16 + c16) * LP + g * 8);
+            acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ad, bx, acc[j], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+
+    const int kk = k0 + wv * 16 + c16;
+    if (kk >= K) return;
+    for (int j = 0; j < BN / 16; j++)
+        for (int e = 0; e < 4; e++) {
+            const int n = n0 + j * 16 + g * 4 + e;
+            if (n < a.cout) a.ws[((int64_t)split * a.cout + n) * K + kk] = acc[j][e];
+        }
+}
+
+// column sums of dy over a chunk of voxels per block: part[blk][n]
+__global__ __launch_bounds__(256) void k_colsum(const uint16_t* dy, int ld, int cout, int M, int mchunk,
+                                                float* part) {
+    const int n = threadIdx.x;
+    if (n >= cout) return;
+    const int mb = blockIdx.x * mchunk, me = min(M, mb + mchunk);
+    float s = 0.f;
+    for (int m = mb; m < me; m++) s += bf2f(dy[(int64_t)m * ld + n]);
+    part[(int64_t)blockIdx.x * cout + n] = s;
+}
+
+// dst[i] = sum_s src[s * len + i], s ascending
+__global__ void k_sum_splits(const float* src, int nsplit, int64_t len, float* dst, int64_t ldd, int64_t rowlen) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= len) return;
+    float s = 0.f;
+    for (int p = 0; p < nsplit; p++) s += src[(int64_t)p * len + i];
+    dst[(i / rowlen) * ldd + i % rowlen] = s;
+}
+
+// ---------------------------------------------------------------------------------------------
+// pooling, upsampling, ReLU backward, features
+
+struct Coarse {
+    int64_t b;
+    int z, y, x;
+};
+
+__device__ __forceinline__ int64_t fine_index(const pvnet_shape& s, int64_t b, int z, int y, int x) {
+    return ((b * s.d + z) * s.h + y) * (int64_t)s.w + x;
+}
+
+__device__ __forceinline__ bool coarse_of(const pvnet_shape& s, int c8n, int64_t i, Coarse& o, int& ch) {
+    const int d2 = s.d / 2, h2 = s.h / 2, w2 = s.w / 2;
+    const int64_t n = (int64_t)s.nb * d2 * h2 * w2 * c8n;
+    if (i >= n) return false;
+    ch = (int)(i % c8n) * 8;
+    int64_t v = i / c8n;
+    o.x = (int)(v % w2);
+    v /= w2;
+    o.y = (int)(v % h2);
+    v /= h2;
+    o.z = (int)(v % d2);
+    o.b = v / d2;
+    return true;
+}
+
+__device__ __forceinline__ int64_t coarse_index(const pvnet_shape& s, const Coarse& o) {
+    return ((o.b * (s.d / 2) + o.z) * (s.h / 2) + o.y) * (int64_t)(s.w / 2) + o.x;
+}
+
+__global__ void k_pool2_fwd(pvnet_shape s, int c, const uint16_t* x, int ldx, uint16_t* y, int ldy) {
+    Coarse o;
+    int ch;
+    if (!coarse_of(s, c / 8, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, o, ch)) return;
+    float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int k = 0; k < 8; k++) {
+        const int64_t f = fine_index(s, o.b, 2 * o.z + (k >> 2), 2 * o.y + ((k >> 1) & 1), 2 * o.x + (k & 1));
+        const uint4 v = *(const uint4*)(x + f * ldx + ch);
+        const uint16_t* p = (const uint16_t*)&v;
+        for (int j = 0; j < 8; j++) acc[j] += bf2f(p[j]);
+    }
+    uint4 out;
+    uint16_t* po = (uint16_t*)&out;
+    for (int j = 0; j < 8; j++) po[j] = f2bf(acc[j] * 0.125f);
+    *(uint4*)(y + coarse_index(s, o) * ldy + ch) = out;
+}
+
+__global__ void k_pool2_bwd(pvnet_shape s, int c, const uint16_t* dy, int lddy, uint16_t* dx, int lddx, int accum) {
+    Coarse o;
+    int ch;
+    if (!coarse_of(s, c / 8, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, o, ch)) return;
+    const uint4 v = *(const uint4*)(dy + coarse_index(s, o) * lddy + ch);
+    const uint16_t* p = (const uint16_t*)&v;
+    float g[8];
+    for (int j = 0; j < 8; j++) g[j] = bf2f(p[j]) * 0.125f;
+    for (int k = 0; k < 8; k++) {
+        const int64_t f = fine_index(s, o.b, 2 * o.z + (k >> 2), 2 * o.y + ((k >> 1) & 1), 2 * o.x + (k & 1));
+        uint4* dst = (uint4*)(dx + f * lddx + ch);
+        uint4 out;
+        uint16_t* po = (uint16_t*)&out;
+        if (accum) {
+            const uint4 old = *dst;
+            const uint16_t* pold = (const uint16_t*)&old;
+            for (int j = 0; j < 8; j++) po[j] = f2bf(bf2f(pold[j]) + g[j]);
+        } else {
+            for (int j = 0; j < 8; j++) po[j] = f2bf(g[j]);
+        }
+        *dst = out;
+    }
+}
+
+__global__ void k_up2_fwd(pvnet_shape s, int c, const uint16_t* x, int ldx, uint16_t* y, int ldy) {
+    Coarse o;
+    int ch;
+    if (!coarse_of(s, c / 8, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, o, ch)) return;
+    const uint4 v = *(const uint4*)(x + coarse_index(s, o) * ldx + ch);
+    for (int k = 0; k < 8; k++) {
+        const int64_t f = fine_index(s, o.b, 2 * o.z + (k >> 2), 2 * o.y + ((k >> 1) & 1), 2 * o.x + (k & 1));
+        *(uint4*)(y + f * ldy + ch) = v;
+    }
+}
+
+__global__ void k_up2_bwd(pvnet_shape s, int c, const uint16_t* dy, int lddy, uint16_t* dx, int lddx) {
+    Coarse o;
+    int ch;
+    if (!coarse_of(s, c / 8, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, o, ch)) return;
+    float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int k = 0; k < 8; k++) {
+        const int64_t f = fine_index(s, o.b, 2 * o.z + (k >> 2), 2 * o.y + ((k >> 1) & 1), 2 * o.x + (k & 1));
+        const uint4 v = *(const uint4*)(dy + f * lddy + ch);
+        const uint16_t* p = (const uint16_t*)&v;
+        for (int j = 0; j < 8; j++) acc[j] += bf2f(p[j]);
+    }
+    uint4 out;
+    uint16_t* po = (uint16_t*)&out;
+    for (int j = 0; j < 8; j++) po[j] = f2bf(acc[j]);
+    *(uint4*)(dx + coarse_index(s, o) * lddx + ch) = out;
+}
+
+__global__ void k_relu_bwd(int64_t m, int c8n, const uint16_t* a, int lda, uint16_t* dx, int lddx) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m * c8n) return;
+    const int64_t v = i / c8n;
+    const int ch = (int)(i % c8n) * 8;
+    const uint4 av = *(const uint4*)(a + v * lda + ch);
+    uint4* dp = (uint4*)(dx + v * lddx + ch);
+    uint4 dv = *dp;
+    const uint16_t* pa = (const uint16_t*)&av;
+    uint16_t* pd = (uint16_t*)&dv;
+    for (int j = 0; j < 8; j++)
+        if (!bf_pos(pa[j])) pd[j] = 0;
+    *dp = dv;
+}
+
+__global__ void k_features(pvnet_shape s, const uint8_t* code, const float* prm, uint16_t* xo) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = (int64_t)s.nb * s.d * s.h * s.w;
+    if (i >= n) return;
+    const int x = (int)(i % s.w), y = (int)((i / s.w) % s.h), z = (int)((i / ((int64_t)s.w * s.h)) % s.d);
+    const int64_t b = i / ((int64_t)s.w * s.h * s.d);
+    const int cd = code[i];
+    const float fl = cd > 0 ? 1.f : 0.f;
+    float f[8];
+    f[0] = fl;
+    f[1] = cd == 2 ? 1.f : 0.f;
+    f[2] = cd == 3 ? 1.f : 0.f;
+    f[3] = fl * (prm[2 * b + 1] - 0.5f) / 0.4f;
+    f[4] = fl * prm[2 * b] / 0.05f;
+    f[5] = fl * ((float)x / (float)max(1, s.w - 1) - 0.5f);
+    f[6] = fl * ((float)y / (float)max(1, s.h - 1) - 0.5f);
+    f[7] = fl * ((float)z / (float)max(1, s.d - 1) - 0.5f);
+    uint4 out;
+    uint16_t* po = (uint16_t*)&out;
+    for (int j = 0; j < 8; j++) po[j] = f2bf(f[j]);
+    *(uint4*)(xo + i * 8) = out;
+}
+
+// ---------------------------------------------------------------------------------------------
+// masked MSE loss: per-block FP64 partials, one final block, then the gradient
+
+constexpr int LOSS_BLOCK = 256;
+
+__global__ __launch_bounds__(LOSS_BLOCK) void k_loss_partial(int64_t M, int nc, const float* y, int ldy,
+                                                             const uint16_t* t, const uint8_t* code,
+                                                             const float* wc, double* part) {
+    __shared__ double ss[LOSS_BLOCK], sc[LOSS_BLOCK];
+    const int64_t i = (int64_t)blockIdx.x * LOSS_BLOCK + threadIdx.x;
+    double e = 0.0, c = 0.0;
+    if (i < M && code[i] > 0) {
+        c = 1.0;
+        for (int k = 0; k < nc; k++) {
+            const float r = y[i * ldy + k] - bf2f(t[i * nc + k]);
+            e += (double)(wc[k] * r * r);
+        }
+    }
+    ss[threadIdx.x] = e;
+    sc[threadIdx.x] = c;
+    __syncthreads();
+    for (int st = LOSS_BLOCK / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) {
+            ss[threadIdx.x] += ss[threadIdx.x + st];
+            sc[threadIdx.x] += sc[threadIdx.x + st];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = ss[0];
+        part[2 * blockIdx.x + 1] = sc[0];
+    }
+}
+
+__global__ __launch_bounds__(LOSS_BLOCK) void k_loss_final(int nblk, int nc, const double* part, double* loss) {
+    __shared__ double ss[LOSS_BLOCK], sc[LOSS_BLOCK];
+    double e = 0.0, c = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += LOSS_BLOCK) {
+        e += part[2 * b];
+        c += part[2 * b + 1];
+    }
+    ss[threadIdx.x] = e;
+    sc[threadIdx.x] = c;
+    __syncthreads();
+    for (int st = LOSS_BLOCK / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) {
+            ss[threadIdx.x] += ss[threadIdx.x + st];
+            sc[threadIdx.x] += sc[threadIdx.x + st];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        loss[0] = sc[0] > 0 ? ss[0] / (sc[0] * nc) : 0.0;
+        loss[1] = sc[0];
+    }
+}
+
+__global__ void k_loss_grad(int64_t M, int nc, const float* y, int ldy, const uint16_t* t, const uint8_t* code,
+                            const float* wc, const double* loss, uint16_t* dy, int lddy) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M) return;
+    const double cnt = loss[1];
+    const float sc = cnt > 0 ? (float)(2.0 / (cnt * nc)) : 0.f;
+    const bool fl = code[i] > 0;
+    for (int k = 0; k < lddy; k++) {
+        float gk = 0.f;
+        if (fl && k < nc) gk = sc * wc[k] * (y[i * ldy + k] - bf2f(t[i * nc + k]));
+        dy[i * lddy + k] = f2bf(gk);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// optimizer and weight packing
+
+__global__ void k_adam(int64_t n, float* p, const float* g, float* m, float* v, float lr, float b1, float b2,
+                       float eps, float c1, float c2) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+__global__ void k_pack(const float* w32, int ntaps, int cin, int cout, int kpad, int kpad_d, uint16_t* wt,
+                       uint16_t* wd) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nt = (int64_t)cout * kpad, nd = (int64_t)cin * kpad_d;
+    const int K = ntaps * cin, Kd = ntaps * cout;
+    if (i < nt) {
+        const int n = (int)(i / kpad), k = (int)(i % kpad);
+        wt[i] = k < K ? f2bf(w32[(int64_t)n * K + k]) : 0;
+    } else if (i < nt + nd) {
+        const int64_t j = i - nt;
+        const int ci = (int)(j / kpad_d), k = (int)(j % kpad_d);
+        uint16_t v = 0;
+        if (k < Kd) {
+            const int tapd = k / cout, co = k % cout;
+            v = f2bf(w32[(int64_t)co * K + (ntaps - 1 - tapd) * cin + ci]);
+        }
+        wd[j] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// host helpers
+
+int64_t voxels(const pvnet_shape& s) { return (int64_t)s.nb * s.d * s.h * s.w; }
+
+bool shape_ok(const pvnet_shape* s) { return s && s->nb > 0 && s->d > 0 && s->h > 0 && s->w > 0; }
+
+bool al16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
+
+int kpad_of(int ntaps, int c) { return (ntaps * c + BK - 1) / BK * BK; }
+
+struct WgradPlan {
+    int ktiles, ntiles, bn, nsplit, mchunk, cblk, cchunk;
+    int64_t ws_main, ws_total;
+};
+
+int wgrad_bn(int cout) { return cout <= 16 ? 16 : (cout <= 32 || cout % 64 != 0) ? 32 : 64; }
+
+WgradPlan plan_wgrad(const pvnet_shape& s, int ntaps, int cin, int cout) {
+    WgradPlan p{};
+    const int64_t M = voxels(s);
+    const int K = ntaps * cin;
+    p.bn = wgrad_bn(cout);
+    p.ktiles = (K + 63) / 64;
+    p.ntiles = (cout + p.bn - 1) / p.bn;
+    const int64_t want = 2048 / (p.ktiles * p.ntiles) + 1;
+    int64_t steps = M / 32;
+    int64_t ns = std::min<int64_t>(std::max<int64_t>(want, 1), std::max<int64_t>(steps, 1));
+    int64_t per = (steps + ns - 1) / ns;
+    p.mchunk = (int)(per * 32);
+    p.nsplit = (int)((M + p.mchunk - 1) / p.mchunk);
+    p.ws_main = (int64_t)p.nsplit * cout * K;
+    p.cchunk = 2048;
+    p.cblk = (int)((M + p.cchunk - 1) / p.cchunk);
+    p.ws_total = p.ws_main + (int64_t)p.cblk * cout;
+    return p;
+}
+
+int blocks_for(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
+
+int pool_args(const pvnet_shape* s, int c, const void* a, int lda, const void* b, int ldb, const char* who) {
+    if (!shape_ok(s) || s->d % 2 || s->h % 2 || s->w % 2) return fail(1, std::string(who) + ": sizes must be even");
+    if (c <= 0 || c % 8 || lda < c || ldb < c || lda % 8 || ldb % 8) return fail(1, std::string(who) + ": bad channels");
+    if (!a || !b || !al16(a) || !al16(b)) return fail(1, std::string(who) + ": null or misaligned pointer");
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* pvnet_last_error(void) { return g_err.c_str(); }
+
+int pvnet_conv_fwd(const pvnet_conv* c, void* stream) {
+    if (!c || !shape_ok(&c->s)) return fail(1, "pvnet_conv_fwd: bad shape");
+    if (c->ntaps != 1 && c->ntaps != 27) return fail(1, "pvnet_conv_fwd: ntaps must be 1 or 27");
+    if (c->cin <= 0 || c->cin % 8 || c->cout <= 0) return fail(1, "pvnet_conv_fwd: cin must be a positive multiple of 8");
+    if (c->kpad != kpad_of(c->ntaps, c->cin)) return fail(1, "pvnet_conv_fwd: kpad must be roundup(ntaps*cin, 32)");
+    if (c->ldx < c->cin || c->ldx % 8 || c->ldy < c->cout) return fail(1, "pvnet_conv_fwd: bad row lengths");
+    if (!c->x || !c->w || !c->y || !al16(c->x) || !al16(c->w)) return fail(1, "pvnet_conv_fwd: null or misaligned pointer");
+    if (c->relu_of && c->ldr < c->cout) return fail(1, "pvnet_conv_fwd: bad relu_of row length");
+    const int64_t M = voxels(c->s);
+    if (M >= (int64_t)1 << 31) return fail(1, "pvnet_conv_fwd: too many voxels");
+    hipStream_t st = (hipStream_t)stream;
+    const int K = c->ntaps * c->cin;
+    const int mt = blocks_for(M, 128);
+    if (c->cout <= 16) {
+        hipLaunchKernelGGL(k_conv_fwd<16>, dim3(mt, 1), dim3(256), 0, st, *c, (int)M, K);
+    } else if (c->cout <= 32 || c->cout % 64 != 0) {
+        hipLaunchKernelGGL(k_conv_fwd<32>, dim3(mt, (c->cout + 31) / 32), dim3(256), 0, st, *c, (int)M, K);
+    } else {
+        hipLaunchKernelGGL(k_conv_fwd<64>, dim3(mt, c->cout / 64), dim3(256), 0, st, *c, (int)M, K);
+    }
+    NET_LAUNCH();
+    return 0;
+}
+
+int64_t pvnet_wgrad_ws_floats(const pvnet_shape* s, int32_t ntaps, int32_t cin, int32_t cout) {
+    if (!shape_ok(s) || cin <= 0 || cout <= 0) return -1;
+    return plan_wgrad(*s, ntaps, cin, cout).ws_total;
+}
+
+int pvnet_conv_wgrad(const pvnet_wgrad* a, void* stream) {
+    if (!a || !shape_ok(&a->s)) return fail(1, "pvnet_conv_wgrad: bad shape");
+    if (a->ntaps != 1 && a->ntaps != 27) return fail(1, "pvnet_conv_wgrad: ntaps must be 1 or 27");
+    if (a->cin <= 0 || a->cin % 8 || a->cout <= 0 || a->cout % 8)
+        return fail(1, "pvnet_conv_wgrad: cin and cout must be positive multiples of 8");
+    if (a->ldx < a->cin || a->ldx % 8 || a->lddy < a->cout || a->lddy % 8) return fail(1, "pvnet_conv_wgrad: bad row lengths");
+    if (!a->x || !a->dy || !a->ws || !a->dw || !al16(a->x) || !al16(a->dy))
+        return fail(1, "pvnet_conv_wgrad: null or misaligned pointer");
+    const int64_t M = voxels(a->s);
+    if (M % 32) return fail(1, "pvnet_conv_wgrad: voxel count must be a multiple of 32");
+    if (M >= (int64_t)1 << 31) return fail(1, "pvnet_conv_wgrad: too many voxels");
+    hipStream_t st = (hipStream_t)stream;
+    const WgradPlan p = plan_wgrad(a->s, a->ntaps, a->cin, a->cout);
+    const int K = a->ntaps * a->cin;
+    dim3 grid(p.ktiles, p.ntiles, p.nsplit);
+    if (p.bn == 16)
+        hipLaunchKernelGGL(k_conv_wgrad<16>, grid, dim3(256), 0, st, *a, (int)M, K, p.mchunk);
+    else if (p.bn == 32)
+        hipLaunchKernelGGL(k_conv_wgrad<32>, grid, dim3(256), 0, st, *a, (int)M, K, p.mchunk);
+    else
+        hipLaunchKernelGGL(k_conv_wgrad<64>, grid, dim3(256), 0, st, *a, (int)M, K, p.mchunk);
+    NET_LAUNCH();
+    const int64_t len = (int64_t)a->cout * K;
+    hipLaunchKernelGGL(k_sum_splits, dim3(blocks_for(len, 256)), dim3(256), 0, st, a->ws, p.nsplit, len, a->dw,
+                       (int64_t)K, (int64_t)K);
+    NET_LAUNCH();
+    if (a->db) {
+        if (a->cout > 256) return fail(1, "pvnet_conv_wgrad: bias gradient supports cout <= 256");
+        float* part = a->ws + p.ws_main;
+        hipLaunchKernelGGL(k_colsum, dim3(p.cblk), dim3(256), 0, st, a->dy, a->lddy, a->cout, (int)M, p.cchunk, part);
+        NET_LAUNCH();
+        hipLaunchKernelGGL(k_sum_splits, dim3(1), dim3(256), 0, st, part, p.cblk, (int64_t)a->cout, a->db,
+                           (int64_t)a->cout, (int64_t)a->cout);
+        NET_LAUNCH();
+    }
+    return 0;
+}
+
+int pvnet_pack_weights(const float* w32, int32_t ntaps, int32_t cin, int32_t cout, uint16_t* wt, uint16_t* wd,
+                       void* stream) {
+    if (!w32 || !wt || !wd || (ntaps != 1 && ntaps != 27) || cin <= 0 || cout <= 0)
+        return fail(1, "pvnet_pack_weights: bad argument");
+    const int kpad = kpad_of(ntaps, cin), kpad_d = kpad_of(ntaps, cout);
+    const int64_t n = (int64_t)cout * kpad + (int64_t)cin * kpad_d;
+    hipLaunchKernelGGL(k_pack, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream, w32, ntaps, cin, cout,
+                       kpad, kpad_d, wt, wd);
+    NET_LAUNCH();
+    return 0;
+}
+
+int pvnet_pool2_fwd(const pvnet_shape* s, int32_t c, const uint16_t* x, int32_t ldx, uint16_t* y, int32_t ldy,
+                    void* stream) {
+    if (int rc = pool_args(s, c, x, ldx, y, ldy, "pvnet_pool2_fwd")) return rc;
+    const int64_t n = voxels(*s) / 8 * (c / 8);
+    hipLaunchKernelGGL(k_pool2_fwd, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream, *s, c, x, ldx, y, ldy);
+    NET_LAUNCH();
+    return 0;
+}
+
+int pvnet_pool2_bwd(const pvnet_shape* s, int32_t c, const uint16_t* dy, int32_t lddy, uint16_t* dx, int32_t lddx,
+                    int32_t accumulate, void* stream) {
+    if (int rc = pool_args(s, c, dy, lddy, dx, lddx, "pvnet_pool2_bwd")) return rc;
+    const int64_t n = voxels(*s) / 8 * (c / 8);
+    hipLaunchKernelGGL(k_pool2_bwd, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream, *s, c, dy, lddy, dx,
+                       lddx, accumulate ? 1 : 0);
+    NET_LAUNCH();
+    return 0;
+}
+
+int pvnet_up2_fwd(const pvnet_shape* s, int32_t c, const uint16_t* x, int32_t ldx, uint16_t* y, int32_t ldy,
+                  void* stream) {
+    if (int rc = pool_args(s, c, x, ldx, y, ldy, "pvnet_up2_fwd")) return rc;
+    const int64_t n = voxels(*s) / 8 * (c / 8);
+    hipLaunchKernelGGL(k_up2_fwd, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream, *s, c, x, ldx, y, ldy);
+    NET_LAUNCH();
+    return 0;
+}
+
+int pvnet_up2_bwd(const pvnet_shape* s, int32_t c, const uint16_t* dy, int32_t lddy, uint16_t* dx, int32_t lddx,
+                  void* stream) {
+    if (int rc = pool_args(s, c, dy, lddy, dx, lddx, "pvnet_up2_bwd")) return rc;
+    const int64_t n = voxels(*s) / 8 * (c / 8);
+    hipLaunchKernelGGL(k_up2_bwd, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream, *s, c, dy, lddy, dx, lddx);
+    NET_LAUNCH();
+    return 0;
+}
+
+int pvnet_relu_bwd(int64_t m, int32_t c, const uint16_t* a, int32_t lda, uint16_t* dx, int32_t lddx, void* stream) {
+    if (m <= 0 || c <= 0 || c % 8 || lda < c || lddx < c || lda % 8 || lddx % 8 || !a || !dx || !al16(a) || !al16(dx))
+        return fail(1, "pvnet_relu_bwd: bad argument");
+    const int64_t n = m * (c / 8);
+    hipLaunchKernelGGL(k_relu_bwd, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream, m, c / 8, a, lda, dx, lddx);
+    NET_LAUNCH();
+    return 0;
+}
+
+int pvnet_features(const pvnet_shape* s, const uint8_t* code, const float* prm, uint16_t* x, void* stream) {
+    if (!shape_ok(s) || !code || !prm || !x || !al16(x)) return fail(1, "pvnet_features: bad argument");
+    const int64_t n = voxels(*s);
+    hipLaunchKernelGGL(k_features, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream, *s, code, prm, x);
+    NET_LAUNCH();
+    return 0;
+}
+
+int64_t pvnet_loss_ws_doubles(int64_t m) { return m > 0 ? 2 * ((m + LOSS_BLOCK - 1) / LOSS_BLOCK) : -1; }
+
+int pvnet_mse_loss(int64_t m, int32_t nc, const float* y, int32_t ldy, const uint16_t* t, const uint8_t* code,
+                   const float* wc, uint16_t* dy, int32_t lddy, double* part, double* loss, void* stream) {
+    if (m <= 0 || nc <= 0 || ldy < nc || lddy < nc || !y || !t || !code || !wc || !dy || !part || !loss)
+        return fail(1, "pvnet_mse_loss: bad argument");
+    hipStream_t st = (hipStream_t)stream;
+    const int nblk = blocks_for(m, LOSS_BLOCK);
+    hipLaunchKernelGGL(k_loss_partial, dim3(nblk), dim3(LOSS_BLOCK), 0, st, m, nc, y, ldy, t, code, wc, part);
+    NET_LAUNCH();
+    hipLaunchKernelGGL(k_loss_final, dim3(1), dim3(LOSS_BLOCK), 0, st, nblk, nc, part, loss);
+    NET_LAUNCH();
+    hipLaunchKernelGGL(k_loss_grad, dim3(blocks_for(m, 256)), dim3(256), 0, st, m, nc, y, ldy, t, code, wc, loss, dy, lddy);
+    NET_LAUNCH();
+    return 0;
+}
+
+int pvnet_adam(int64_t n, float* p, const float* g, float* m, float* v, float lr, float b1, float b2, float eps,
+               int32_t t, void* stream) {
+    if (n <= 0 || !p || !g || !m || !v || t < 1) return fail(1, "pvnet_adam: bad argument");
+    const float c1 = 1.f - powf(b1, (float)t), c2 = 1.f - powf(b2, (float)t);
+    hipLaunchKernelGGL(k_adam, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream, n, p, g, m, v, lr, b1, b2,
+                       eps, c1, c2);
+    NET_LAUNCH();
+    return 0;
+}
+
+}  // extern "C"

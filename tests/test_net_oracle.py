@@ -1,0 +1,182 @@
+"""Pins of the network oracle (oracle/pvnet_oracle.c) on the CPU: each check is independent of the
+formula it tests (a library routine, an adjoint identity, finite differences or a closed form)."""
+import numpy as np
+import pytest
+from scipy import ndimage
+
+from pv_dec_original_amd.warmnet import UNetSpec
+from tests import net_oracle as O
+
+RNG = np.random.default_rng(1234)
+
+
+def _case(shape, cin, cout, ntaps):
+    M = O.nvox(shape)
+    x = RNG.standard_normal((M, cin))
+    w = RNG.standard_normal((cout, ntaps * cin))
+    b = RNG.standard_normal(cout)
+    return x, w, b
+
+
+@pytest.mark.parametrize("ntaps", [27, 1])
+def test_conv_fwd_matches_scipy_correlate(ntaps):
+    shape, cin, cout = (2, 4, 5, 6), 8, 3
+    x, w, b = _case(shape, cin, cout, ntaps)
+    y = O.conv_fwd(shape, ntaps, cin, cout, x, w, b)
+    xs = x.reshape(*shape, cin)
+    ref = np.zeros((*shape, cout))
+    for n in range(cout):
+        for bb in range(shape[0]):
+            acc = np.full(shape[1:], b[n])
+            for ci in range(cin):
+                if ntaps == 27:
+                    k = w[n].reshape(3, 3, 3, cin)[..., ci]  # taps ordered (dz, dy, dx)
+                    acc += ndimage.correlate(xs[bb, ..., ci], k, mode="constant", cval=0.0)
+                else:
+                    acc += xs[bb, ..., ci] * w[n, ci]
+            ref[bb, ..., n] = acc
+    assert np.allclose(y, ref.reshape(-1, cout), rtol=1e-12, atol=1e-12)
+
+
+def test_conv_relu():
+    shape, cin, cout = (1, 4, 4, 4), 8, 4
+    x, w, b = _case(shape, cin, cout, 27)
+    y = O.conv_fwd(shape, 27, cin, cout, x, w, b)
+    yr = O.conv_fwd(shape, 27, cin, cout, x, w, b, relu=True)
+    assert np.array_equal(yr, np.maximum(y, 0.0)) and (y < 0).any()
+
+
+@pytest.mark.parametrize("ntaps", [27, 1])
+def test_conv_backward_adjoint_identities(ntaps):
+    """<conv(x; w), dy> = <x, dgrad(dy; w)> = <w, wgrad(x, dy)> (bias off), db = sum dy."""
+    shape, cin, cout = (2, 4, 6, 4), 8, 5
+    x, w, _ = _case(shape, cin, cout, ntaps)
+    dy = RNG.standard_normal((O.nvox(shape), cout))
+    y = O.conv_fwd(shape, ntaps, cin, cout, x, w)
+    dx = O.conv_dgrad(shape, ntaps, cin, cout, dy, w)
+    dw, db = O.conv_wgrad(shape, ntaps, cin, cout, x, dy)
+    lhs = np.sum(y * dy)
+    assert np.isclose(lhs, np.sum(x * dx), rtol=1e-11)
+    assert np.isclose(lhs, np.sum(w * dw), rtol=1e-11)
+    assert np.allclose(db, dy.sum(axis=0), rtol=1e-12)
+
+
+def test_conv_dgrad_finite_difference():
+    shape, cin, cout = (1, 3, 4, 4), 8, 2
+    x, w, b = _case(shape, cin, cout, 27)
+    dy = RNG.standard_normal((O.nvox(shape), cout))
+    dx = O.conv_dgrad(shape, 27, cin, cout, dy, w)
+    for idx in [(0, 0), (7, 3), (O.nvox(shape) - 1, 7), (20, 5)]:
+        e = 1e-6
+        xp, xm = x.copy(), x.copy()
+        xp[idx] += e
+        xm[idx] -= e
+        fd = (np.sum(O.conv_fwd(shape, 27, cin, cout, xp, w, b) * dy) -
+              np.sum(O.conv_fwd(shape, 27, cin, cout, xm, w, b) * dy)) / (2 * e)
+        assert np.isclose(fd, dx[idx], rtol=1e-6, atol=1e-8)
+
+
+def test_pool_and_upsample_adjoints_and_identities():
+    shape, c = (2, 4, 6, 8), 8
+    x = RNG.standard_normal((O.nvox(shape), c))
+    yc = RNG.standard_normal((O.nvox(O.coarse(shape)), c))
+    p = O.pool2_fwd(shape, c, x)
+    assert np.isclose(np.sum(p * yc), np.sum(x * O.pool2_bwd(shape, c, yc)), rtol=1e-12)
+    u = O.up2_fwd(shape, c, yc)
+    assert np.isclose(np.sum(u * x), np.sum(yc * O.up2_bwd(shape, c, x)), rtol=1e-12)
+    # pooling an upsampled field returns it; pooling a constant gives the constant
+    assert np.allclose(O.pool2_fwd(shape, c, u), yc, rtol=1e-14)
+    assert np.allclose(O.pool2_fwd(shape, c, np.full_like(x, 3.5)), 3.5)
+    # the mean over each 2x2x2 block, computed by reshaping
+    xs = x.reshape(2, 2, 2, 3, 2, 4, 2, c).mean(axis=(2, 4, 6)).reshape(-1, c)
+    assert np.allclose(p, xs, rtol=1e-13)
+
+
+def test_features_known_answer():
+    shape = (2, 2, 2, 4)
+    code = np.zeros(shape, dtype=np.uint8)
+    code[0, 0, 0, 0] = 2
+    code[0, 1, 1, 3] = 3
+    code[1, 0, 1, 2] = 1
+    prm = np.array([[0.05, 0.9], [0.01, 0.5]])
+    f = O.features(shape, code, prm).reshape(*shape, 8)
+    assert np.allclose(f[0, 0, 0, 0], [1, 1, 0, 1, 1, -0.5, -0.5, -0.5])
+    assert np.allclose(f[0, 1, 1, 3], [1, 0, 1, 1, 1, 0.5, 0.5, 0.5])
+    assert np.allclose(f[1, 0, 1, 2], [1, 0, 0, 0, 0.2, 2 / 3 - 0.5, 0.5, -0.5])
+    assert np.count_nonzero(f.reshape(-1, 8).any(axis=1)) == 3
+
+
+def test_mse_loss_value_and_gradient():
+    M, nc = 50, 4
+    code = RNG.integers(0, 4, M).astype(np.uint8)
+    y = RNG.standard_normal((M, 8))
+    t = RNG.standard_normal((M, nc))
+    wc = np.array([1.0, 2.0, 0.5, 1.0])
+    L, dy = O.mse_loss(y, t, code, wc)
+    fl = code > 0
+    ref = np.sum(wc * (y[fl, :nc] - t[fl]) ** 2) / (fl.sum() * nc)
+    assert np.isclose(L, ref, rtol=1e-13)
+    assert np.all(dy[~fl] == 0) and np.all(dy[:, nc:] == 0)
+    for idx in [(int(np.argmax(fl)), 1), (int(np.nonzero(fl)[0][-1]), 3)]:
+        e = 1e-6
+        yp, ym = y.copy(), y.copy()
+        yp[idx] += e
+        ym[idx] -= e
+        fd = (O.mse_loss(yp, t, code, wc)[0] - O.mse_loss(ym, t, code, wc)[0]) / (2 * e)
+        assert np.isclose(fd, dy[idx], rtol=1e-6)
+    assert O.mse_loss(np.concatenate([t, np.zeros((M, 4))], 1), t, code, wc)[0] == 0.0
+
+
+def test_adam_first_step_is_signed_lr():
+    """At t = 1 the bias-corrected moments are g and g^2: each parameter moves by lr*g/(|g|+eps)."""
+    n = 64
+    p = RNG.standard_normal(n)
+    g = RNG.standard_normal(n)
+    p1, m1, v1 = O.adam(p, g, np.zeros(n), np.zeros(n), 1e-3, 0.9, 0.999, 1e-8, 1)
+    assert np.allclose(p1, p - 1e-3 * g / (np.abs(g) + 1e-8), rtol=1e-12, atol=1e-15)
+    assert np.allclose(m1, 0.1 * g) and np.allclose(v1, 0.001 * g * g)
+
+
+def test_bf16_round_known_values():
+    v = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, -2.5, 1.0 + 2 ** -9, 3.0e38])
+    r = O.bf16_round(v)
+    # bf16 spacing at 1 is 2^-7: 1 + 2^-8 is a tie that goes to 1 (even), 1 + 3*2^-8 one that goes
+    # to 1 + 2^-6 (even), 1 + 2^-9 rounds down
+    assert r[0] == 1.0 and r[1] == 1.0 and r[2] == 1.0 + 2 ** -6 and r[3] == -2.5 and r[4] == 1.0
+
+
+def test_oracle_unet_gradient_finite_difference():
+    """The FP64 oracle U-Net's backward against central differences of its forward loss."""
+    spec = UNetSpec(levels=(8, 16), cin=8, cout=8)
+    params = spec.init_params(seed=3).astype(np.float64)
+    params[:] += 0.01 * RNG.standard_normal(params.size)  # non-zero biases too
+    shape = (1, 4, 4, 6)
+    code = RNG.integers(0, 4, shape).astype(np.uint8)
+    prm = np.array([[0.03, 0.7]])
+    t = RNG.standard_normal((O.nvox(shape), 4))
+    wc = np.ones(4)
+    net = O.OracleUNet(spec, params, emulate_bf16=False)
+
+    def loss(pv):
+        n = O.OracleUNet(spec, pv, emulate_bf16=False)
+        return O.mse_loss(n.forward(code, prm), t, code, wc)[0]
+
+    L, dY = O.mse_loss(net.forward(code, prm), t, code, wc)
+    g = net.backward(dY)
+    rng = np.random.default_rng(7)
+    for lay in spec.layers:
+        for idx in list(rng.integers(lay.offset, lay.boff, 2)) + [lay.boff]:
+            e = 1e-6
+            pp, pm = params.copy(), params.copy()
+            pp[idx] += e
+            pm[idx] -= e
+            fd = (loss(pp) - loss(pm)) / (2 * e)
+            assert np.isclose(fd, g[idx], rtol=1e-4, atol=1e-9), (lay.name, idx, fd, g[idx])
+
+
+def test_unet_spec_default_size():
+    spec = UNetSpec()
+    assert 1.7e6 < spec.nparams < 2.1e6
+    assert [lay.name for lay in spec.layers][:2] == ["e0a", "e0b"] and spec.layers[-1].name == "head"
+    with pytest.raises(ValueError):
+        spec.check_shape((1, 96, 96, 20))
