@@ -10,6 +10,10 @@ Output: one JSON line.  value = fluid-node updates per second / 1e6 (MLUPS) over
 inputs resident in HBM.  roofline.achieved = fluid nodes * 2 * Q * 8 bytes / average launch time.
 cpu_baseline = the CPU oracle (a restatement of the reference solver, push streaming) on a bounded
 number of steps of the same workload, single-threaded like one reference MPI rank.
+
+"learned" (on by default, --learned 0 to skip): wall time to converged wall shear stress on the
+1024 x 512 x 512 aneurysm, the plain FP64 path from rest against FP16 shifted storage started from
+the learned warm start (pv_dec_original_amd/learned_bench.py).
 """
 import argparse
 import json
@@ -92,6 +96,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-bytes", type=int, default=1 << 30,
                     help="size of the calibration copy kernel (0 = skip)")
+    ap.add_argument("--learned", type=int, default=1,
+                    help="also measure wall time to converged WSS on the large aneurysm: plain FP64 "
+                         "from rest against FP16 storage from the learned warm start (1 = on)")
+    ap.add_argument("--learned-ablations", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -202,6 +210,14 @@ def main():
         },
         "cpu_baseline": cpu,
     }
+    if args.learned:
+        del sd, sim  # free the Y-bifurcation's device memory first
+        from pv_dec_original_amd import learned_bench
+        lb = learned_bench.run(ablations=args.learned_ablations, log=log)
+        out["metric"] = "MLUPS + % HBM roofline; wall time to converged WSS (<=1% vs FP64) plain vs chosen"
+        out["learned"] = {k: v for k, v in lb.items() if k != "runs"}
+        out["learned"]["runs"] = {k: {kk: vv for kk, vv in v.items() if kk != "history"}
+                                  for k, v in lb["runs"].items()}
     print(json.dumps(out), flush=True)
 
 

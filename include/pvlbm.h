@@ -166,7 +166,10 @@ int pvlbm_set_inlet_profile(pvlbm_handle h, const double* u);
 
 /* Restart from f = f_eq(rho, u) (warm start; no reference counterpart).  rho and u are dense box
  * arrays (u as three planes x, y, z); rho == NULL means rest (f = w, the reference start,
- * Node.cpp:30-37).  Resets the step counter, the WSS accumulators and the stale populations. */
+ * Node.cpp:30-37).  Resets the step counter, the WSS accumulators and the stale populations;
+ * those (populations of links without an upstream node, which the reference never writes after
+ * its start from rest) take the rest value w_i in both cases, so a start from given fields
+ * converges to the same steady state as a start from rest. */
 int pvlbm_init_fields(pvlbm_handle h, const double* rho, const double* u);
 
 /* Every how many iterations WSS and its accumulators are evaluated (reference: every iteration,

@@ -95,12 +95,19 @@ int pvnet_up2_bwd(const pvnet_shape* s, int32_t c, const uint16_t* dy, int32_t l
 int pvnet_relu_bwd(int64_t m, int32_t c, const uint16_t* a, int32_t lda, uint16_t* dx, int32_t lddx,
                    void* stream);
 
+/* Euclidean distance (in voxels) from each voxel to the nearest solid voxel (code 0) of its box,
+ * exact: the separable squared-distance transform of Saito and Toriwaki (1994), one pass per
+ * axis; capped at 255 (a box without solid voxels).  Box sizes up to PVNET_EDT_MAX per axis. */
+#define PVNET_EDT_MAX 512
+int pvnet_wall_distance(const pvnet_shape* s, const uint8_t* code, float* dist, void* stream);
+
 /* Input features (8 bf16 channels) from a geometry code per voxel (0 solid, 1 fluid, 2 inlet
- * face, 3 outlet face) and per-box parameters prm[b] = {u_mean, tau}:
+ * face, 3 outlet face), its wall distance (pvnet_wall_distance) and per-box parameters
+ * prm[b] = {u_mean, tau, inlet_area} (inlet_area: number of inlet-face voxels):
  *   c0 fluid, c1 inlet, c2 outlet, c3 fluid*(tau-0.5)/0.4, c4 fluid*u_mean/0.05,
- *   c5 fluid*(x/(w-1) - 0.5), c6 fluid*(y/(h-1) - 0.5), c7 fluid*(z/(d-1) - 0.5). */
-int pvnet_features(const pvnet_shape* s, const uint8_t* code, const float* prm, uint16_t* x,
-                   void* stream);
+ *   c5 fluid*(x/(w-1) - 0.5), c6 fluid*dist/16, c7 fluid*inlet_area/(pi*16^2). */
+int pvnet_features(const pvnet_shape* s, const uint8_t* code, const float* prm, const float* dist,
+                   uint16_t* x, void* stream);
 
 /* Masked mean-squared error over fluid voxels (code > 0) and the first nc channels of y (FP32,
  * row length ldy) against t (bf16 [m][nc]) with channel weights wc[nc]:

@@ -149,8 +149,25 @@ void onet_up2_bwd(const int32_t s[4], int c, const double* dy, double* dx) {
     }
 }
 
+/* distance from each voxel to the nearest solid voxel (code 0) of its box, by brute force over
+ * every solid voxel of the box; 255 when the box has none */
+void onet_wall_distance(const int32_t s[4], const uint8_t* code, double* dist) {
+    const int64_t per = (int64_t)s[1] * s[2] * s[3];
+    FOR_VOXELS(s) {
+        const int64_t m = VOX(s);
+        double best = 1e300;
+        for (int64_t q = 0; q < per; q++) {
+            if (code[b * per + q] != 0) continue;
+            const int qz = (int)(q / ((int64_t)s[2] * s[3])), qy = (int)((q / s[3]) % s[2]), qx = (int)(q % s[3]);
+            const double d2 = (double)(qz - iz) * (qz - iz) + (double)(qy - iy) * (qy - iy) + (double)(qx - ix) * (qx - ix);
+            if (d2 < best) best = d2;
+        }
+        dist[m] = best > 255.0 * 255.0 ? 255.0 : sqrt(best);
+    }
+}
+
 /* the eight input features of a geometry code volume (see pvnet_features) */
-void onet_features(const int32_t s[4], const uint8_t* code, const double* prm, double* out) {
+void onet_features(const int32_t s[4], const uint8_t* code, const double* prm, const double* dist, double* out) {
     FOR_VOXELS(s) {
         const int64_t m = VOX(s);
         const int cd = code[m];
@@ -159,11 +176,11 @@ void onet_features(const int32_t s[4], const uint8_t* code, const double* prm, d
         f[0] = fl;
         f[1] = cd == 2 ? 1.0 : 0.0;
         f[2] = cd == 3 ? 1.0 : 0.0;
-        f[3] = fl * (prm[2 * b + 1] - 0.5) / 0.4;
-        f[4] = fl * prm[2 * b] / 0.05;
+        f[3] = fl * (prm[3 * b + 1] - 0.5) / 0.4;
+        f[4] = fl * prm[3 * b] / 0.05;
         f[5] = fl * ((double)ix / (double)(s[3] > 1 ? s[3] - 1 : 1) - 0.5);
-        f[6] = fl * ((double)iy / (double)(s[2] > 1 ? s[2] - 1 : 1) - 0.5);
-        f[7] = fl * ((double)iz / (double)(s[1] > 1 ? s[1] - 1 : 1) - 0.5);
+        f[6] = fl * dist[m] / 16.0;
+        f[7] = fl * prm[3 * b + 2] / (3.14159265358979323846 * 256.0);
     }
 }
 

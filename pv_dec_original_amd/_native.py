@@ -6,6 +6,15 @@ raised.  Build with ``python -m pv_dec_original_amd.build`` or ``__graft_entry__
 import ctypes as C
 import os
 
+# One HIP runtime per process.  PyTorch (device memory and streams of the warm-start network)
+# ships its own libamdhip64/libhsa-runtime64; the libraries here link the system ones under the
+# same soname.  Loaded first, torch's runtime serves every library of the process; loaded after
+# ours, a second runtime would come up beside the first and torch would see no device.
+try:  # noqa: SIM105
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 PVLBM_OK = 0
@@ -105,7 +114,8 @@ PVNET_API = [
     ("pvnet_up2_fwd", C.c_int, [_sp, C.c_int32, _vp, C.c_int32, _vp, C.c_int32, _vp]),
     ("pvnet_up2_bwd", C.c_int, [_sp, C.c_int32, _vp, C.c_int32, _vp, C.c_int32, _vp]),
     ("pvnet_relu_bwd", C.c_int, [C.c_int64, C.c_int32, _vp, C.c_int32, _vp, C.c_int32, _vp]),
-    ("pvnet_features", C.c_int, [_sp, _vp, _vp, _vp, _vp]),
+    ("pvnet_wall_distance", C.c_int, [_sp, _vp, _vp, _vp]),
+    ("pvnet_features", C.c_int, [_sp, _vp, _vp, _vp, _vp, _vp]),
     ("pvnet_loss_ws_doubles", C.c_int64, [C.c_int64]),
     ("pvnet_mse_loss", C.c_int, [C.c_int64, C.c_int32, _vp, C.c_int32, _vp, _vp, _vp, _vp, C.c_int32,
                                  _vp, _vp, _vp]),

@@ -359,11 +359,47 @@ __global__ void k_relu_bwd(int64_t m, int c8n, const uint16_t* a, int lda, uint1
     *dp = dv;
 }
 
-__global__ void k_features(pvnet_shape s, const uint8_t* code, const float* prm, uint16_t* xo) {
+// squared-distance transform along one axis: g(i) = min_j (g_in(j) + (i - j)^2); the first pass
+// starts from g_in = 0 on solid voxels and "infinite" elsewhere.  One thread per line.
+constexpr float EDT_INF = 1e30f;
+
+__global__ void k_edt_pass(pvnet_shape s, int axis, const uint8_t* code, float* g) {
+    const int64_t L = (int64_t)s.nb * s.d * s.h * s.w;
+    const int n = axis == 0 ? s.w : axis == 1 ? s.h : s.d;
+    const int64_t nlines = L / n;
+    const int64_t line = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (line >= nlines) return;
+    int64_t base, stride;
+    if (axis == 0) {
+        base = line * s.w;
+        stride = 1;
+    } else if (axis == 1) {
+        const int64_t x = line % s.w, bz = line / s.w;
+        base = bz * (int64_t)s.h * s.w + x;
+        stride = s.w;
+    } else {
+        const int64_t xy = line % ((int64_t)s.w * s.h), b = line / ((int64_t)s.w * s.h);
+        base = b * (int64_t)s.d * s.h * s.w + xy;
+        stride = (int64_t)s.w * s.h;
+    }
+    float in[PVNET_EDT_MAX];
+    for (int j = 0; j < n; j++)
+        in[j] = axis == 0 ? (code[base + j * stride] == 0 ? 0.f : EDT_INF) : g[base + j * stride];
+    for (int i = 0; i < n; i++) {
+        float best = EDT_INF;
+        for (int j = 0; j < n; j++) {
+            const float dj = (float)(i - j);
+            best = fminf(best, in[j] + dj * dj);
+        }
+        g[base + i * stride] = axis == 2 ? fminf(sqrtf(best), 255.f) : best;
+    }
+}
+
+__global__ void k_features(pvnet_shape s, const uint8_t* code, const float* prm, const float* dist, uint16_t* xo) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t n = (int64_t)s.nb * s.d * s.h * s.w;
     if (i >= n) return;
-    const int x = (int)(i % s.w), y = (int)((i / s.w) % s.h), z = (int)((i / ((int64_t)s.w * s.h)) % s.d);
+    const int x = (int)(i % s.w);
     const int64_t b = i / ((int64_t)s.w * s.h * s.d);
     const int cd = code[i];
     const float fl = cd > 0 ? 1.f : 0.f;
@@ -371,11 +407,11 @@ __global__ void k_features(pvnet_shape s, const uint8_t* code, const float* prm,
     f[0] = fl;
     f[1] = cd == 2 ? 1.f : 0.f;
     f[2] = cd == 3 ? 1.f : 0.f;
-    f[3] = fl * (prm[2 * b + 1] - 0.5f) / 0.4f;
-    f[4] = fl * prm[2 * b] / 0.05f;
+    f[3] = fl * (prm[3 * b + 1] - 0.5f) / 0.4f;
+    f[4] = fl * prm[3 * b] / 0.05f;
     f[5] = fl * ((float)x / (float)max(1, s.w - 1) - 0.5f);
-    f[6] = fl * ((float)y / (float)max(1, s.h - 1) - 0.5f);
-    f[7] = fl * ((float)z / (float)max(1, s.d - 1) - 0.5f);
+    f[6] = fl * dist[i] / 16.f;
+    f[7] = fl * prm[3 * b + 2] / 804.247719f;  // pi * 16^2
     uint4 out;
     uint16_t* po = (uint16_t*)&out;
     for (int j = 0; j < 8; j++) po[j] = f2bf(f[j]);
@@ -666,10 +702,27 @@ int pvnet_relu_bwd(int64_t m, int32_t c, const uint16_t* a, int32_t lda, uint16_
     return 0;
 }
 
-int pvnet_features(const pvnet_shape* s, const uint8_t* code, const float* prm, uint16_t* x, void* stream) {
-    if (!shape_ok(s) || !code || !prm || !x || !al16(x)) return fail(1, "pvnet_features: bad argument");
+int pvnet_wall_distance(const pvnet_shape* s, const uint8_t* code, float* dist, void* stream) {
+    if (!shape_ok(s) || !code || !dist) return fail(1, "pvnet_wall_distance: bad argument");
+    if (s->d > PVNET_EDT_MAX || s->h > PVNET_EDT_MAX || s->w > PVNET_EDT_MAX)
+        return fail(1, "pvnet_wall_distance: box sizes are limited to PVNET_EDT_MAX");
     const int64_t n = voxels(*s);
-    hipLaunchKernelGGL(k_features, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream, *s, code, prm, x);
+    const int len[3] = {s->w, s->h, s->d};
+    for (int axis = 0; axis < 3; axis++) {
+        const int64_t lines = n / len[axis];
+        hipLaunchKernelGGL(k_edt_pass, dim3(blocks_for(lines, 64)), dim3(64), 0, (hipStream_t)stream, *s, axis, code,
+                           dist);
+        NET_LAUNCH();
+    }
+    return 0;
+}
+
+int pvnet_features(const pvnet_shape* s, const uint8_t* code, const float* prm, const float* dist, uint16_t* x,
+                   void* stream) {
+    if (!shape_ok(s) || !code || !prm || !dist || !x || !al16(x)) return fail(1, "pvnet_features: bad argument");
+    const int64_t n = voxels(*s);
+    hipLaunchKernelGGL(k_features, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream, *s, code, prm, dist,
+                       x);
     NET_LAUNCH();
     return 0;
 }

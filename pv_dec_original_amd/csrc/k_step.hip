@@ -188,7 +188,19 @@ __device__ __forceinline__ bool node_update(const StepParams& p, double (&red)[4
         init_populations<Q>(p, n, f);
         if (bnd) cl = p.bclose[bslot];
     }
-    if (bnd) save_stale<Q, ST>(p, nb, nmiss, f);
+    if (bnd) {
+        if (init && p.init_rho) {
+            // a start from given fields keeps the rest value w in the populations of links
+            // without an upstream node: the reference starts from rest and never writes them
+            // again, so any other value would move the steady state
+            R fw[Q];
+#pragma unroll
+            for (int d = 0; d < Q; d++) fw[d] = R(Lat<Q>::w[d]);
+            save_stale<Q, ST>(p, nb, nmiss, fw);
+        } else {
+            save_stale<Q, ST>(p, nb, nmiss, f);
+        }
+    }
 
     R rho, u[3];
     moments(f, rho, u);

@@ -1,8 +1,9 @@
 """The warm-start network: a 3D convolutional U-Net on voxel geometry, run by libpvnet.so.
 
 The reference solver always starts from rest (Simulation.cpp:13-15: every population at its
-weight).  This network maps a geometry code volume (0 solid, 1 fluid, 2 inlet face, 3 outlet face)
-and the inlet parameters (u_mean, tau) to an initial field, normalised as
+weight).  This network maps a geometry code volume (0 solid, 1 fluid, 2 inlet face, 3 outlet face),
+its wall distance and the inlet parameters (u_mean, tau, inlet area) to an initial field,
+normalised as
     y[..., 0:3] = u / u_mean,   y[..., 3] = (rho - 1) / u_mean,
 from which the solver runs on (pvlbm_init_fields); the solver judges the field and starting from
 rest stays the fallback.
@@ -107,8 +108,13 @@ class UNetSpec:
         nb, d, h, w = shape
         if min(shape) <= 0 or d % self.divisor or h % self.divisor or w % self.divisor:
             raise ValueError(f"box sizes must be multiples of {self.divisor}, got {shape}")
+
+    def check_train_shape(self, shape):
+        """The weight-gradient kernels also need a multiple of 32 voxels at the coarsest level."""
+        self.check_shape(shape)
+        nb, d, h, w = shape
         if (nb * d * h * w // self.divisor ** 3) % 32:
-            raise ValueError("the coarsest level must hold a multiple of 32 voxels")
+            raise ValueError("training needs a multiple of 32 voxels at the coarsest level")
 
 
 def _torch():
@@ -181,6 +187,7 @@ class UNet:
         b = {"shape": shape, "M": M,
              "s": [N.NetShape(nb, d >> k, h >> k, w >> k) for k in range(L)],
              "X0": bf(M[0], IN_CH), "Y": torch.zeros((M[0], OUT_CH), dtype=torch.float32, device=dev),
+             "dist": torch.zeros(M[0], dtype=torch.float32, device=dev),
              "dY": bf(M[0], OUT_CH)}
         for k in range(L - 1):
             ccat = lv[k + 1] + lv[k]
@@ -232,13 +239,18 @@ class UNet:
     # -- forward / backward -----------------------------------------------------------------
 
     def forward(self, code, prm):
-        """code: uint8 device tensor (nb, d, h, w); prm: float32 device tensor (nb, 2) of
-        (u_mean, tau).  Returns the FP32 output (nb*d*h*w, 8) (a buffer reused by later calls)."""
+        """code: uint8 device tensor (nb, d, h, w); prm: float32 device tensor (nb, 3) of
+        (u_mean, tau, inlet_area).  Returns the FP32 output (nb*d*h*w, 8) (a buffer reused by
+        later calls)."""
         b = self.buffers(tuple(code.shape))
+        if tuple(prm.shape) != (code.shape[0], 3) or prm.dtype != self.torch.float32 or not prm.is_contiguous():
+            raise ValueError("prm must be a contiguous float32 tensor (nb, 3)")
         NL = N.net()
         lv, L, S = self.spec.levels, len(self.spec.levels), b["s"]
         by = self.spec.by_name
-        N.net_check(NL.pvnet_features(C.byref(S[0]), _ptr(code), _ptr(prm), _ptr(b["X0"]), self._st()))
+        N.net_check(NL.pvnet_wall_distance(C.byref(S[0]), _ptr(code), _ptr(b["dist"]), self._st()))
+        N.net_check(NL.pvnet_features(C.byref(S[0]), _ptr(code), _ptr(prm), _ptr(b["dist"]), _ptr(b["X0"]),
+                                      self._st()))
         x, ldx = b["X0"], IN_CH
         for k in range(L - 1):
             ccat = lv[k + 1] + lv[k]
@@ -273,6 +285,7 @@ class UNet:
     def backward(self):
         """Gradients of every parameter from the dY buffer (set by loss()) into self.g."""
         b = self._bufs
+        self.spec.check_train_shape(b["shape"])
         NL = N.net()
         lv, L, S = self.spec.levels, len(self.spec.levels), b["s"]
         by = self.spec.by_name
@@ -318,13 +331,3 @@ class UNet:
                                        lr, b1, b2, eps, self.t, self._st()))
         self.pack()
 
-    def predict(self, code, prm):
-        """Denormalised (u, rho) of a batch: u (nb, d, h, w, 3), rho (nb, d, h, w), FP32 device
-        tensors."""
-        y = self.forward(code, prm)
-        nb, d, h, w = code.shape
-        y = y.view(nb, d, h, w, OUT_CH)
-        um = prm[:, 0].view(nb, 1, 1, 1)
-        u = y[..., 0:3] * um.unsqueeze(-1)
-        rho = 1.0 + y[..., 3] * um
-        return u, rho

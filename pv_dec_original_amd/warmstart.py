@@ -9,11 +9,12 @@ CHECK steps falls below TOL; its (u, rho) is the target, normalised as u / u_mea
 (rho - 1) / u_mean.
 
 Warm start: the network sees the geometry code on a grid of at most NET_MAX voxels per axis (the
-solver grid coarsened by an integer factor s when larger), with (u_mean, tau) mapped to the
-coarse grid at equal Reynolds number; its output is upsampled trilinearly to the solver grid and
-denormalised with the solver's u_mean.  The field is checked (finite, |u| below UMAX, rho within
-RHO_RANGE) before pvlbm_init_fields sets f = f_eq(rho, u); a rejected field, or a run that turns
-non-finite afterwards, restarts from rest, the reference start.
+solver grid resampled by a factor s when larger), with (u_mean, tau) mapped to the coarse grid at
+equal Reynolds number; its output is upsampled trilinearly to the solver grid and
+denormalised with the solver's u_mean.  A field that is not finite, or far out of range (a speed
+above REJECT_U, a density outside REJECT_RHO), is rejected and the solver starts from rest, the
+reference start; otherwise speeds are clipped to UMAX and densities into RHO_CLIP before
+pvlbm_init_fields sets f = f_eq(rho, u).
 """
 import time
 from concurrent.futures import ThreadPoolExecutor
@@ -27,9 +28,11 @@ RAMP = 500
 CHECK = 250
 TOL = 2e-4
 MAX_STEPS = 12000
-UMAX = 0.35
-RHO_RANGE = (0.7, 1.4)
-NET_MAX = 96
+UMAX = 0.3  # speeds above are clipped to it (lattice units; stability of BGK)
+RHO_CLIP = (0.8, 1.25)  # densities are clipped into this range
+REJECT_U = 1.0  # a field with a faster node, or a non-finite value, is rejected
+REJECT_RHO = (0.5, 2.0)
+NET_MAX = 192
 
 
 def sample_params(seed):
@@ -80,6 +83,7 @@ class Sample:
     seed: int
     u_mean: float
     tau: float
+    inlet_area: float
     code: np.ndarray  # (n, n, n) uint8
     target: np.ndarray  # (n, n, n, 4) float32, normalised
     steps: int
@@ -116,7 +120,9 @@ def solve_sample(seed, n=96, tol=TOL, max_steps=MAX_STEPS):
         target[..., 2] = u[2] / um
         target[..., 3] = (rho - 1.0) / um
         target[~fluid] = 0.0
-    return Sample(int(seed), um, ta, code_from_kinds(kinds), target, steps, status, time.perf_counter() - t0)
+    code = code_from_kinds(kinds)
+    return Sample(int(seed), um, ta, float(np.count_nonzero(code == 2)), code, target, steps, status,
+                  time.perf_counter() - t0)
 
 
 def generate(seeds, n=96, workers=4, log=None, **kw):
@@ -145,44 +151,7 @@ def load_weights(path):
 
 
 # ---------------------------------------------------------------------------------------------
-# training
-
-def train(net, codes, targets, prms, epochs=1, batch=4, lr=1e-3, lr_end=1e-5, seed=0, log=None,
-          time_budget=None, val=None):
-    """Adam over minibatches of device-resident samples.
-    codes: uint8 (S, d, h, w); targets: bf16 (S, d*h*w, 4); prms: float32 (S, 2) — torch device
-    tensors.  Cosine learning-rate decay from lr to lr_end over the planned steps.  Returns the
-    per-step training losses."""
-    import torch
-    S = codes.shape[0]
-    steps_per_epoch = S // batch
-    total = max(1, steps_per_epoch * epochs)
-    rng = np.random.default_rng(seed)
-    losses = []
-    t0 = time.perf_counter()
-    step = 0
-    for ep in range(epochs):
-        order = rng.permutation(S)
-        for b in range(steps_per_epoch):
-            idx = torch.from_numpy(order[b * batch:(b + 1) * batch].copy()).to(codes.device)
-            code = codes.index_select(0, idx).contiguous()
-            tgt = targets.index_select(0, idx).reshape(-1, targets.shape[-1]).contiguous()
-            prm = prms.index_select(0, idx).contiguous()
-            cur = lr_end + 0.5 * (lr - lr_end) * (1 + np.cos(np.pi * step / total))
-            net.forward(code, prm)
-            loss = net.loss(tgt, code)
-            net.backward()
-            net.adam_step(float(cur))
-            step += 1
-            if step % 50 == 0 or step == total:
-                losses.append((step, float(loss[0])))
-                if log:
-                    log(f"step {step}/{total} epoch {ep} loss {losses[-1][1]:.4e} lr {cur:.2e} "
-                        f"{time.perf_counter() - t0:.0f}s")
-            if time_budget and time.perf_counter() - t0 > time_budget:
-                return losses
-    return losses
-
+# evaluation (training itself: train_warmstart.py)
 
 def evaluate(net, codes, targets, prms, batch=4):
     """Mean masked MSE and the relative L2 error of u over the fluid voxels, per sample."""
@@ -217,14 +186,14 @@ def coarse_params(u_mean, tau, s):
     return uc, tc
 
 
-def coarsen_code(code, s):
-    """Code of a grid coarsened by the integer factor s: a coarse voxel is fluid when the fine
-    node at its centre is; inlet/outlet faces are the fluid coarse voxels on the x planes that
-    hold fine inlet/outlet nodes."""
-    if s == 1:
+def resample_code(code, cshape):
+    """Code on a coarser grid cshape by nearest sampling at the coarse voxel centres; the fluid
+    voxels of the first and last x planes are the inlet and outlet faces when the fine grid has
+    them there."""
+    if tuple(cshape) == tuple(code.shape):
         return code.copy()
-    nz, ny, nx = code.shape
-    c = code[s // 2::s, s // 2::s, s // 2::s][: nz // s, : ny // s, : nx // s].copy()
+    idx = [np.minimum(((np.arange(c) + 0.5) * n / c).astype(np.int64), n - 1) for c, n in zip(cshape, code.shape)]
+    c = code[np.ix_(*idx)].copy()
     c[c > 1] = 1
     fl = c > 0
     if (code[:, :, 0] == 2).any():
@@ -234,14 +203,13 @@ def coarsen_code(code, s):
     return c
 
 
-def _trilinear_up(field, s, shape):
-    """Values at fine nodes from values at coarse voxel centres (coarse voxel k covers fine nodes
-    k*s .. k*s + s - 1, centre at k*s + (s - 1)/2); clamped at the edges."""
+def _trilinear_up(field, shape):
+    """Values at the fine nodes of `shape` from values at the centres of the coarse voxels of
+    `field` (coarse voxel k spans fine coordinates [k, k+1) * n/c); clamped at the edges."""
     out = field
     for axis, n in enumerate(shape):
-        x = (np.arange(n) - (s - 1) / 2.0) / s
         nc = field.shape[axis]
-        x = np.clip(x, 0, nc - 1)
+        x = np.clip((np.arange(n) + 0.5) * nc / n - 0.5, 0, nc - 1)
         i0 = np.floor(x).astype(np.int64)
         i1 = np.minimum(i0 + 1, nc - 1)
         wgt = (x - i0).reshape([-1 if a == axis else 1 for a in range(out.ndim)])
@@ -263,9 +231,9 @@ class WarmStart:
         """(rho, u) on the grid of `code` (d, h, w), u as (3, d, h, w), numpy FP64."""
         import torch
         d, h, w = code.shape
-        self.net.spec.check_shape((1, d, h, w))
         cd = torch.from_numpy(np.ascontiguousarray(code)).to("cuda").reshape(1, d, h, w)
-        prm = torch.tensor([[u_mean, tau]], dtype=torch.float32, device="cuda")
+        prm = torch.tensor([[u_mean, tau, float(np.count_nonzero(code == 2))]], dtype=torch.float32,
+                           device="cuda")
         y = self.net.forward(cd, prm)
         torch.cuda.synchronize()
         y = y.view(d, h, w, -1).cpu().numpy().astype(np.float64)
@@ -273,27 +241,34 @@ class WarmStart:
         rho = 1.0 + y[..., 3] * u_mean
         return rho, u
 
+    def net_grid(self, shape):
+        """The network's grid for a solver grid: the solver grid itself when it fits (at most
+        NET_MAX voxels per axis, sizes divisible by the network's divisor), else a grid with the
+        longest axis at NET_MAX and the others scaled alike, each rounded to the divisor.
+        Returns (coarse shape, scale s = fine / coarse along the longest axis)."""
+        div = self.net.spec.divisor
+        if max(shape) <= NET_MAX and all(n % div == 0 for n in shape):
+            return tuple(shape), 1.0
+        s = max(shape) / float(NET_MAX // div * div)
+        s = max(s, 1.0)
+        return tuple(max(div, int(round(n / s / div)) * div) for n in shape), s
+
     def fields(self, kinds, u_mean, tau):
         """Initial (rho, u) on the solver grid of `kinds` (nz, ny, nx)."""
         code = code_from_kinds(kinds)
-        div = self.net.spec.divisor
-        s = max(1, -(-max(kinds.shape) // NET_MAX))
-        cshape = [max(div, (n // s) // div * div) for n in kinds.shape]
-        c = coarsen_code(code, s)
-        cc = np.zeros(cshape, dtype=np.uint8)
-        sl = tuple(slice(0, min(a, b)) for a, b in zip(cshape, c.shape))
-        cc[sl] = c[sl]
+        cshape, s = self.net_grid(kinds.shape)
+        cc = resample_code(code, cshape)
         uc, tc = coarse_params(u_mean, tau, s) if s > 1 else (u_mean, tau)
         rho_c, u_c = self.predict(cc, uc, tc)
         # at equal Reynolds number u / u_mean is the same field, and rho - 1 (viscous drop
         # nu u L / R^2 or inertial u^2 alike) scales with u_mean^2 / u_c^2
         u_c = u_c * (u_mean / uc)
         rho_c = 1.0 + (rho_c - 1.0) * (u_mean / uc) ** 2
-        if s == 1:
+        if cshape == tuple(kinds.shape):
             rho, u = rho_c, u_c
         else:
-            rho = _trilinear_up(rho_c, s, kinds.shape)
-            u = np.stack([_trilinear_up(u_c[k], s, kinds.shape) for k in range(3)])
+            rho = _trilinear_up(rho_c, kinds.shape)
+            u = np.stack([_trilinear_up(u_c[k], kinds.shape) for k in range(3)])
         fluid = kinds != N.SOLID
         rho = np.where(fluid, rho, 1.0)
         u = np.where(fluid[None], u, 0.0)
@@ -301,14 +276,22 @@ class WarmStart:
 
 
 def field_ok(rho, u, fluid):
-    """The check a predicted field must pass before the solver starts from it."""
+    """The check a predicted field must pass before the solver starts from it: finite, and no
+    speed or density so far out that clipping would not be a repair."""
     r = rho[fluid]
     v = u[:, fluid]
     if not (np.isfinite(r).all() and np.isfinite(v).all()):
         return False
-    if np.abs(v).max(initial=0.0) > UMAX:
+    if np.sqrt((v * v).sum(axis=0)).max(initial=0.0) > REJECT_U:
         return False
-    return bool(r.min(initial=1.0) >= RHO_RANGE[0] and r.max(initial=1.0) <= RHO_RANGE[1])
+    return bool(r.min(initial=1.0) >= REJECT_RHO[0] and r.max(initial=1.0) <= REJECT_RHO[1])
+
+
+def clip_field(rho, u):
+    """Speeds above UMAX scaled down to UMAX, densities clipped into RHO_CLIP."""
+    sp = np.sqrt((u * u).sum(axis=0))
+    f = np.where(sp > UMAX, UMAX / np.maximum(sp, 1e-300), 1.0)
+    return np.clip(rho, *RHO_CLIP), u * f[None]
 
 
 def start(sim, sd, warm, u_mean, tau):
@@ -321,7 +304,43 @@ def start(sim, sd, warm, u_mean, tau):
         except (ValueError, N.NativeError):
             rho = u = None
         if rho is not None and field_ok(rho, u, kinds != N.SOLID):
+            rho, u = clip_field(rho, u)
             sim.init_fields(rho, u)
             return "warm"
     sim.init_fields()
     return "rest"
+
+
+# ---------------------------------------------------------------------------------------------
+# convergence of wall shear stress: the measure the warm start is judged by
+
+def reference_wss(sim, fluid, tol=1e-6, max_steps=40000):
+    """WSS magnitude per next-to-wall node of the converged steady state (run from rest with the
+    inlet ramp until the relative change of u over CHECK steps is below tol)."""
+    sim.init_fields()
+    rho, u, steps, change, status = run_to_steady(sim, fluid, tol=tol, max_steps=max_steps)
+    sim.set_wss_every(1)
+    sim.step(1)
+    return sim.wss()[:, 3].copy(), steps, status
+
+
+def steps_to_wss(sim, ref, tol=0.01, check=100, max_steps=40000, ramp=RAMP, timed=True):
+    """Iterations (and the kernels' milliseconds) until the relative L2 difference of the WSS
+    magnitude from ref is at most tol, sampled every `check` iterations from the current state.
+    Returns (steps or None, error, ms)."""
+    sim.set_wss_every(check)
+    it = 0
+    ms = 0.0
+    err = np.inf
+    nref = max(np.linalg.norm(ref), 1e-300)
+    while it < max_steps:
+        r = sim.step(check, ramp_scales(it, check, ramp), timed=timed)
+        ms += r or 0.0
+        it += check
+        w = sim.wss()[:, 3]
+        if not np.isfinite(w).all():
+            return None, np.inf, ms
+        err = float(np.linalg.norm(w - ref) / nref)
+        if err <= tol:
+            return it, err, ms
+    return None, err, ms

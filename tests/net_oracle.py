@@ -32,7 +32,8 @@ def load():
             "onet_pool2_bwd": (None, [_i32, i, dp, dp]),
             "onet_up2_fwd": (None, [_i32, i, dp, dp]),
             "onet_up2_bwd": (None, [_i32, i, dp, dp]),
-            "onet_features": (None, [_i32, vp, dp, dp]),
+            "onet_wall_distance": (None, [_i32, vp, dp]),
+            "onet_features": (None, [_i32, vp, dp, dp, dp]),
             "onet_mse_loss": (C.c_double, [C.c_int64, i, dp, i, dp, vp, dp, dp]),
             "onet_adam": (None, [C.c_int64, dp, dp, dp, dp, C.c_double, C.c_double, C.c_double,
                                  C.c_double, i]),
@@ -117,10 +118,20 @@ def up2_bwd(shape, c, dy):
     return dx
 
 
-def features(shape, code, prm):
+def wall_distance(shape, code):
+    out = np.zeros(nvox(shape))
+    code = np.ascontiguousarray(code, dtype=np.uint8)
+    load().onet_wall_distance(_s(shape), code.ctypes.data_as(C.c_void_p), _p(out))
+    return out
+
+
+def features(shape, code, prm, dist=None):
+    """prm: (nb, 3) of (u_mean, tau, inlet_area); dist: wall distance (computed when None)."""
     out = np.zeros((nvox(shape), 8))
     code = np.ascontiguousarray(code, dtype=np.uint8)
-    load().onet_features(_s(shape), code.ctypes.data_as(C.c_void_p), _p(_c(prm)), _p(out))
+    if dist is None:
+        dist = wall_distance(shape, code)
+    load().onet_features(_s(shape), code.ctypes.data_as(C.c_void_p), _p(_c(prm)), _p(_c(dist)), _p(out))
     return out
 
 
@@ -169,7 +180,8 @@ class OracleUNet:
         nb, d, h, w = code.shape
         S = [(nb, d >> k, h >> k, w >> k) for k in range(L)]
         a = {"S": S}
-        a["X0"] = self._r(features(S[0], code, prm))
+        dist = wall_distance(S[0], code).astype(np.float32).astype(np.float64)
+        a["X0"] = self._r(features(S[0], code, np.asarray(prm, dtype=np.float32).astype(np.float64), dist))
         x = a["X0"]
         by = sp.by_name
         for k in range(L - 1):

@@ -188,16 +188,21 @@ def test_pool_up_relu_features_loss_adam(T):
     N.net_check(NL.pvnet_relu_bwd(M, c, _p(ad), c, _p(gd), c, st))
     T.cuda.synchronize()
     assert np.array_equal(_np(gd), g * (a > 0))
-    # features
+    # wall distance and features
     code = RNG.integers(0, 4, shape).astype(np.uint8)
-    prm = np.array([[0.03, 0.7], [0.012, 0.58]])
+    code[RNG.random(shape) < 0.5] = 1
+    prm = np.array([[0.03, 0.7, 12.0], [0.012, 0.58, 30.0]])
     cd = T.from_numpy(code).to("cuda")
+    distd = T.zeros(M, dtype=T.float32, device="cuda")
+    N.net_check(NL.pvnet_wall_distance(C.byref(S), _p(cd), _p(distd), st))
     fd = T.zeros((M, 8), dtype=T.bfloat16, device="cuda")
     prmd = _dev(T, prm, T.float32)
-    N.net_check(NL.pvnet_features(C.byref(S), _p(cd), _p(prmd), _p(fd), st))
+    N.net_check(NL.pvnet_features(C.byref(S), _p(cd), _p(prmd), _p(distd), _p(fd), st))
     T.cuda.synchronize()
-    _close_bf16(_np(fd), O.bf16_round(O.features(shape, code, prm.astype(np.float32).astype(np.float64))),
-                "features")
+    dref = O.wall_distance(shape, code)
+    assert np.allclose(distd.cpu().numpy(), dref, rtol=1e-6, atol=1e-6)
+    _close_bf16(_np(fd), O.bf16_round(O.features(shape, code, prm.astype(np.float32).astype(np.float64),
+                                                 dref.astype(np.float32).astype(np.float64))), "features")
     # loss
     y = RNG.standard_normal((M, 8)).astype(np.float32).astype(np.float64)
     t = _rbf((M, 4))
@@ -228,6 +233,28 @@ def test_pool_up_relu_features_loss_adam(T):
     assert np.allclose(_np(pd), pr, rtol=1e-6, atol=1e-7)
 
 
+def test_wall_distance_large_box(T):
+    """The separable transform on a 96^3 vessel-like box against the brute-force oracle on a
+    sample of voxels (the oracle is exact; every voxel would take minutes)."""
+    from pv_dec_original_amd import _native as N
+    NL = N.net()
+    n = 96
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    code = ((((y - 40) ** 2 + (z - 50) ** 2) < 14 ** 2) | (((x - 60) ** 2 + (y - 30) ** 2) < 9 ** 2)).astype(np.uint8)
+    code = code.reshape(1, n, n, n)
+    cd = T.from_numpy(code).to("cuda")
+    distd = T.zeros(n ** 3, dtype=T.float32, device="cuda")
+    N.net_check(NL.pvnet_wall_distance(C.byref(N.NetShape(1, n, n, n)), _p(cd), _p(distd), _st(T)))
+    T.cuda.synchronize()
+    got = distd.cpu().numpy()
+    solid = np.argwhere(code[0] == 0)
+    fl = np.flatnonzero(code.reshape(-1))
+    for m in list(RNG.choice(fl, 150)) + list(RNG.integers(0, n ** 3, 50)):
+        p = np.array([m // (n * n), (m // n) % n, m % n])
+        ref = np.sqrt(((solid - p) ** 2).sum(axis=1).min())
+        assert abs(got[m] - ref) < 1e-4, (p, got[m], ref)
+
+
 def test_entry_points_reject_bad_arguments(T):
     from pv_dec_original_amd import _native as N
     NL = N.net()
@@ -246,8 +273,8 @@ def _unet_case(T, levels, shape, seed):
     params = (params + 0.02 * rng.standard_normal(params.size)).astype(np.float32)
     code = rng.integers(0, 4, shape).astype(np.uint8)
     code[rng.random(shape) < 0.3] = 0
-    prm = np.array([[0.01 + 0.04 * rng.random(), 0.55 + 0.35 * rng.random()] for _ in range(shape[0])],
-                   dtype=np.float32)
+    prm = np.array([[0.01 + 0.04 * rng.random(), 0.55 + 0.35 * rng.random(), float(np.count_nonzero(code[b] == 2))]
+                    for b in range(shape[0])], dtype=np.float32)
     t = O.bf16_round(rng.standard_normal((O.nvox(shape), 4)))
     return spec, params, code, prm, t, UNet(spec, params)
 

@@ -98,12 +98,28 @@ def test_features_known_answer():
     code[0, 0, 0, 0] = 2
     code[0, 1, 1, 3] = 3
     code[1, 0, 1, 2] = 1
-    prm = np.array([[0.05, 0.9], [0.01, 0.5]])
-    f = O.features(shape, code, prm).reshape(*shape, 8)
-    assert np.allclose(f[0, 0, 0, 0], [1, 1, 0, 1, 1, -0.5, -0.5, -0.5])
-    assert np.allclose(f[0, 1, 1, 3], [1, 0, 1, 1, 1, 0.5, 0.5, 0.5])
-    assert np.allclose(f[1, 0, 1, 2], [1, 0, 0, 0, 0.2, 2 / 3 - 0.5, 0.5, -0.5])
+    prm = np.array([[0.05, 0.9, 20.0], [0.01, 0.5, 0.0]])
+    dist = np.arange(O.nvox(shape), dtype=np.float64) / 4.0
+    f = O.features(shape, code, prm, dist).reshape(*shape, 8)
+    a = 20.0 / (np.pi * 256.0)
+    assert np.allclose(f[0, 0, 0, 0], [1, 1, 0, 1, 1, -0.5, 0.0, a])
+    assert np.allclose(f[0, 1, 1, 3], [1, 0, 1, 1, 1, 0.5, 15 / 4 / 16, a])
+    assert np.allclose(f[1, 0, 1, 2], [1, 0, 0, 0, 0.2, 2 / 3 - 0.5, 22 / 4 / 16, 0])
     assert np.count_nonzero(f.reshape(-1, 8).any(axis=1)) == 3
+
+
+def test_wall_distance_matches_scipy_edt():
+    """Brute-force distance to the nearest solid voxel against scipy's exact EDT, per box."""
+    shape = (2, 7, 9, 10)
+    code = (RNG.random(shape) < 0.85).astype(np.uint8) * RNG.integers(1, 4, shape).astype(np.uint8)
+    code[1] = 1
+    code[1, 3, 4, 5] = 0  # one solid voxel in the second box
+    d = O.wall_distance(shape, code).reshape(shape)
+    for b in range(2):
+        ref = ndimage.distance_transform_edt(code[b] > 0)
+        assert np.allclose(d[b], ref, rtol=0, atol=1e-12)
+    allfluid = np.ones((1, 2, 2, 2), dtype=np.uint8)
+    assert np.all(O.wall_distance((1, 2, 2, 2), allfluid) == 255.0)
 
 
 def test_mse_loss_value_and_gradient():
@@ -152,7 +168,7 @@ def test_oracle_unet_gradient_finite_difference():
     params[:] += 0.01 * RNG.standard_normal(params.size)  # non-zero biases too
     shape = (1, 4, 4, 6)
     code = RNG.integers(0, 4, shape).astype(np.uint8)
-    prm = np.array([[0.03, 0.7]])
+    prm = np.array([[0.03, 0.7, 5.0]])
     t = RNG.standard_normal((O.nvox(shape), 4))
     wc = np.ones(4)
     net = O.OracleUNet(spec, params, emulate_bf16=False)

@@ -1,0 +1,123 @@
+"""Host logic of the learned warm start (no device): field checks, the fallback to rest, the
+equal-Reynolds mapping to the network grid, code coarsening and trilinear upsampling."""
+import numpy as np
+
+from pv_dec_original_amd import _native as N
+from pv_dec_original_amd import warmstart as W
+
+
+class _Sim:
+    def __init__(self):
+        self.calls = []
+
+    def init_fields(self, rho=None, u=None):
+        self.calls.append("rest" if rho is None else "warm")
+
+
+class _SD:
+    def __init__(self, kinds):
+        self._k = kinds
+
+    def kinds(self):
+        return self._k
+
+
+class _Net:
+    def __init__(self, rho, u):
+        self.rho, self.u = rho, u
+
+    def fields(self, kinds, u_mean, tau):
+        return self.rho, self.u
+
+
+def _kinds():
+    k = np.zeros((4, 4, 8), dtype=np.uint8)
+    k[1:3, 1:3, :] = N.FLUID
+    k[1:3, 1:3, 0] = N.KINLET
+    k[1:3, 1:3, -1] = N.KOUTLET
+    return k
+
+
+def test_field_checks_and_fallback_to_rest():
+    k = _kinds()
+    fl = k != N.SOLID
+    rho = np.ones(k.shape)
+    u = np.zeros((3,) + k.shape)
+    u[0][fl] = 0.05
+    assert W.field_ok(rho, u, fl)
+    for bad in ("nan", "fast", "dense"):
+        r, v = rho.copy(), u.copy()
+        if bad == "nan":
+            v[1, 1, 1, 3] = np.nan
+        elif bad == "fast":
+            v[0, 2, 2, 4] = 1.5
+        else:
+            r[1, 2, 5] = 2.5
+        assert not W.field_ok(r, v, fl), bad
+        sim = _Sim()
+        assert W.start(sim, _SD(k), _Net(r, v), 0.03, 0.7) == "rest"
+        assert sim.calls == ["rest"]
+    sim = _Sim()
+    assert W.start(sim, _SD(k), _Net(rho, u), 0.03, 0.7) == "warm" and sim.calls == ["warm"]
+    # moderate excursions are clipped, not rejected
+    r, v = rho.copy(), u.copy()
+    v[0, 2, 2, 4] = 0.6
+    r[1, 2, 5] = 1.5
+    assert W.field_ok(r, v, fl)
+    rc, vc = W.clip_field(r, v)
+    assert np.isclose(np.sqrt((vc ** 2).sum(axis=0)).max(), W.UMAX) and rc.max() == W.RHO_CLIP[1]
+    assert np.array_equal(vc[:, 1, 1, 3], v[:, 1, 1, 3])
+    sim = _Sim()
+    assert W.start(sim, _SD(k), None, 0.03, 0.7) == "rest" and sim.calls == ["rest"]
+    # values on solid nodes do not matter
+    v = u.copy()
+    v[:, ~fl] = np.nan
+    assert W.field_ok(rho, v, fl)
+
+
+def test_coarse_params_keep_reynolds_number():
+    for u, tau, s in [(0.03, 0.6, 5), (0.01, 0.9, 2), (0.05, 0.55, 3)]:
+        uc, tc = W.coarse_params(u, tau, s)
+        assert 0.01 <= uc <= 0.05 and 0.55 <= tc <= 0.9
+        re_f = s * u / ((tau - 0.5) / 3.0)
+        re_c = uc / ((tc - 0.5) / 3.0)
+        if 0.01 < uc < 0.05 and 0.55 < tc < 0.9:
+            assert np.isclose(re_f, re_c, rtol=1e-12)
+    # unclamped case: (0.03, 0.9, 2) keeps it exactly
+    uc, tc = W.coarse_params(0.03, 0.9, 2)
+    assert np.isclose(2 * 0.03 / (0.4 / 3), uc / ((tc - 0.5) / 3), rtol=1e-12)
+
+
+def test_resample_code_and_trilinear_upsampling():
+    code = np.zeros((8, 8, 12), dtype=np.uint8)
+    code[2:6, 2:6, :] = 1
+    code[2:6, 2:6, 0] = 2
+    code[2:6, 2:6, -1] = 3
+    c = W.resample_code(code, (4, 4, 6))
+    assert c.shape == (4, 4, 6)
+    assert (c[:, :, 0][c[:, :, 0] > 0] == 2).all() and (c[:, :, -1][c[:, :, -1] > 0] == 3).all()
+    assert set(np.unique(c[:, :, 1:-1])) <= {0, 1} and (c[1:3, 1:3, 1:-1] == 1).all()
+    assert W.resample_code(code, code.shape).tolist() == code.tolist()
+    # trilinear interpolation reproduces a linear field between the coarse centres, for
+    # integer and non-integer ratios alike
+    for shape, cshape in [((8, 8, 12), (4, 4, 6)), ((10, 12, 16), (4, 5, 6))]:
+        cc = [(np.arange(c) + 0.5) * n / c - 0.5 for c, n in zip(cshape, shape)]
+        zc, yc, xc = np.meshgrid(*cc, indexing="ij")
+        lin = 0.5 + 0.1 * xc - 0.2 * yc + 0.05 * zc
+        up = W._trilinear_up(lin, shape)
+        z, y, x = np.meshgrid(*[np.arange(n) for n in shape], indexing="ij")
+        ref = 0.5 + 0.1 * x - 0.2 * y + 0.05 * z
+        inner = tuple(slice(int(np.ceil(n / c)), n - int(np.ceil(n / c))) for c, n in zip(cshape, shape))
+        assert np.allclose(up[inner], ref[inner], rtol=0, atol=1e-12)
+
+
+def test_network_grid_choice():
+    from types import SimpleNamespace
+    ws = W.WarmStart.__new__(W.WarmStart)
+    ws.net = SimpleNamespace(spec=SimpleNamespace(divisor=8))
+    assert ws.net_grid((96, 96, 96)) == ((96, 96, 96), 1.0)
+    # the large aneurysm box: the longest axis maps to 192 voxels, radius 40 to 7.5
+    cshape, s = ws.net_grid((512, 512, 1024))
+    assert cshape == (96, 96, 192) and np.isclose(s, 1024 / 192)
+    cshape, s = ws.net_grid((100, 50, 300))
+    assert all(c % 8 == 0 for c in cshape) and max(cshape) == 192
