@@ -172,6 +172,17 @@ int pvlbm_set_inlet_profile(pvlbm_handle h, const double* u);
  * converges to the same steady state as a start from rest. */
 int pvlbm_init_fields(pvlbm_handle h, const double* rho, const double* u);
 
+/* As pvlbm_init_fields from a coarse FP32 field in device memory: rho_c[cz][cy][cx] and
+ * u_c[3][cz][cy][cx] (x fastest; cdims = {cx, cy, cz}) cover the grid, coarse voxel (i, j, k)
+ * centred at fine coordinates ((i + 0.5) nx / cx - 0.5, ...); each node takes the trilinear
+ * sample (clamped at the edges), its speed clipped to umax and its density into
+ * [rho_lo, rho_hi].  Nodes whose sample is not finite start at rest; their number is returned in
+ * *nonfinite.  No counterpart in the reference (it always starts from rest, Simulation.cpp:13-15):
+ * the start from the learned warm-start network, without a host copy of the full-size field. */
+int pvlbm_init_fields_coarse(pvlbm_handle h, const float* rho_c, const float* u_c,
+                             const int32_t cdims[3], double umax, double rho_lo, double rho_hi,
+                             int64_t* nonfinite);
+
 /* Every how many iterations WSS and its accumulators are evaluated (reference: every iteration,
  * Simulation.cpp:585-586).  Default 1. */
 int pvlbm_set_wss_every(pvlbm_handle h, int every);

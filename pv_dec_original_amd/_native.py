@@ -88,6 +88,8 @@ PVLBM_API = [
     ("pvlbm_get_kinds", C.c_int, [C.c_void_p, _u8]),
     ("pvlbm_set_inlet_profile", C.c_int, [C.c_void_p, _d]),
     ("pvlbm_init_fields", C.c_int, [C.c_void_p, _d, _d]),
+    ("pvlbm_init_fields_coarse", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_int32),
+                                           C.c_double, C.c_double, C.c_double, _i64]),
     ("pvlbm_set_wss_every", C.c_int, [C.c_void_p, C.c_int]),
     ("pvlbm_step", C.c_int, [C.c_void_p, C.c_int, _d]),
     ("pvlbm_step_timed", C.c_int, [C.c_void_p, C.c_int, _d, _d]),

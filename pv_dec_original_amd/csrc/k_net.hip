@@ -161,56 +161,105 @@ __global__ __launch_bounds__(256) void k_conv_fwd(pvnet_conv a, int M, int K) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// weight gradient: block tile BN output channels x 64 k-values, reduction over its voxel chunk
+// weight gradient: block tile BN output channels x 64 k-values, reduction over its voxel chunk in
+// LDS fills of 128 voxels.  Threads 0..127 each load an 8-voxel x 8-channel block of the gathered
+// input (eight 16-byte loads), transpose it in registers and write eight 16-byte LDS rows of 8
+// voxels; threads 128.. do the same for dy.  The MFMA reduction index is the voxel.
+
+// 8x8 transpose of bf16: r[i] = row i (8 channels, channel c in 16-bit half c); o[c] = column c
+// (rows 0..7 in its halves)
+__device__ __forceinline__ void transpose8(const uint4 (&r)[8], uint4 (&o)[8]) {
+    const uint32_t* rw = reinterpret_cast<const uint32_t*>(r);
+    uint32_t* ow = reinterpret_cast<uint32_t*>(o);
+#pragma unroll
+    for (int c = 0; c < 8; c++)
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint32_t lo = rw[(2 * k) * 4 + (c >> 1)], hi = rw[(2 * k + 1) * 4 + (c >> 1)];
+            ow[c * 4 + k] = (c & 1) ? ((lo >> 16) | (hi & 0xffff0000u)) : ((lo & 0xffffu) | (hi << 16));
+        }
+}
 
 template <int BN>
 __global__ __launch_bounds__(256) void k_conv_wgrad(pvnet_wgrad a, int M, int K, int mchunk) {
-    __shared__ __attribute__((aligned(16))) uint16_t As[BN * LP];  // dy^T  [n][m]
-    __shared__ __attribute__((aligned(16))) uint16_t Bs[64 * LP];  // x^T   [k][m]
+    constexpr int BMW = 128;  // voxels per LDS fill
+    constexpr int PW = BMW + 8;  // LDS row pitch in bf16 (272 bytes)
+    __shared__ __attribute__((aligned(16))) uint16_t As[BN * PW];  // dy^T  [n][m]
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[64 * PW];  // x^T   [k][m]
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int g = lane >> 4, c16 = lane & 15;
     const int k0 = blockIdx.x * 64, n0 = blockIdx.y * BN, split = blockIdx.z;
     const int mb = split * mchunk, me = min(M, mb + mchunk);
     const int d = a.s.d, h = a.s.h, w = a.s.w;
 
-    const int mm = t >> 3, c8 = (t & 7) * 8;
-    const int k = k0 + c8;
-    const bool kin = k < K;
+    // roles: x blocks (t < 128): voxel group mg, k chunk kc; dy blocks (128 <= t < 128 + 2 BN)
+    const bool xrole = t < 128;
+    const int u = t - 128;
+    const bool drole = !xrole && u < 2 * BN;
+    const int mg = xrole ? (t >> 3) : drole ? u / (BN / 8) : 0;
+    const int kc = t & 7, nc = drole ? u % (BN / 8) : 0;
+    const int k = k0 + kc * 8;
+    const bool kin = xrole && k < K;
     const int tap = kin ? k / a.cin : 0;
     const int ci = k - tap * a.cin;
     int dz, dy, dx;
     tap_offset(a.ntaps, tap, dz, dy, dx);
     const int64_t off = ((int64_t)dz * h + dy) * w + dx;
-    const bool drow = c8 < BN && n0 + c8 < a.cout;
+    const bool dcol = drole && n0 + nc * 8 < a.cout;
 
     f32x4 acc[BN / 16];
     for (int j = 0; j < BN / 16; j++) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    uint4 vd, vx;
+    uint4 r[8];
     auto load = [&](int mbase) {
-        const int m = mbase + mm;
-        vd = make_uint4(0, 0, 0, 0);
-        if (drow) vd = *(const uint4*)(a.dy + (int64_t)m * a.lddy + n0 + c8);
-        const int x = m % w, tq = m / w, y = tq % h, z = (tq / h) % d;
-        const int z2 = z + dz, y2 = y + dy, x2 = x + dx;
-        const bool in = kin && (unsigned)z2 < (unsigned)d && (unsigned)y2 < (unsigned)h && (unsigned)x2 < (unsigned)w;
-        vx = make_uint4(0, 0, 0, 0);
-        if (in) vx = *(const uint4*)(a.x + ((int64_t)m + off) * a.ldx + ci);
+        const int m0 = mbase + mg * 8;
+        if (xrole) {
+            int x = m0 % w, tq = m0 / w, y = tq % h, z = (tq / h) % d;
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                const int m = m0 + i;
+                const int z2 = z + dz, y2 = y + dy, x2 = x + dx;
+                const bool in = kin && m < me && (unsigned)z2 < (unsigned)d && (unsigned)y2 < (unsigned)h &&
+                                (unsigned)x2 < (unsigned)w;
+                r[i] = make_uint4(0, 0, 0, 0);
+                if (in) r[i] = *(const uint4*)(a.x + ((int64_t)m + off) * a.ldx + ci);
+                if (++x == w) {  // next voxel
+                    x = 0;
+                    if (++y == h) {
+                        y = 0;
+                        if (++z == d) z = 0;
+                    }
+                }
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                const int m = m0 + i;
+                r[i] = make_uint4(0, 0, 0, 0);
+                if (dcol && m < me) r[i] = *(const uint4*)(a.dy + (int64_t)m * a.lddy + n0 + nc * 8);
+            }
+        }
     };
 
     if (mb < me) load(mb);
-    for (int mbase = mb; mbase < me; mbase += 32) {
-        const uint16_t* pd = (const uint16_t*)&vd;
-        const uint16_t* px = (const uint16_t*)&vx;
-        if (c8 < BN)
-            for (int j = 0; j < 8; j++) As[(c8 + j) * LP + mm] = pd[j];
-        for (int j = 0; j < 8; j++) Bs[(c8 + j) * LP + mm] = px[j];
+    for (int mbase = mb; mbase < me; mbase += BMW) {
+        if (xrole || drole) {
+            uint4 o[8];
+            transpose8(r, o);
+            uint16_t* dst = xrole ? Bs + (kc * 8) * PW + mg * 8 : As + (nc * 8) * PW + mg * 8;
+#pragma unroll
+            for (int c = 0; c < 8; c++) *(uint4*)(dst + c * PW) = o[c];
+        }
         __syncthreads();
-        if (mbase + 32 < me) load(mbase + 32);
-        const bf16x8 bx = *(const bf16x8*)(Bs + (wv * 16 + c16) * LP + g * 8);
-        for (int j = 0; j < BN / 16; j++) {
-            const bf16x8 ad = *(const bf16x8*)(As + (j * 16 + c16) * LP + g * 8);
-            acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ad, bx, acc[j], 0, 0, 0);
+        if (mbase + BMW < me) load(mbase + BMW);
+#pragma unroll
+        for (int kk = 0; kk < BMW / 32; kk++) {
+            const bf16x8 bx = *(const bf16x8*)(Bs + (wv * 16 + c16) * PW + kk * 32 + g * 8);
+#pragma unroll
+            for (int j = 0; j < BN / 16; j++) {
+                const bf16x8 ad = *(const bf16x8*)(As + (j * 16 + c16) * PW + kk * 32 + g * 8);
+                acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ad, bx, acc[j], 0, 0, 0);
+            }
         }
         __syncthreads();
     }
@@ -224,15 +273,30 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(pvnet_wgrad a, int M, int K,
         }
 }
 
-// column sums of dy over a chunk of voxels per block: part[blk][n]
+// column sums of dy over a chunk of voxels per block: part[blk][n].  Thread t sums the 8-channel
+// group t % g over the rows r = t / g, t / g + R, ... (g = cout / 8, R = 256 / g); the R partial
+// rows are then added in ascending order.
 __global__ __launch_bounds__(256) void k_colsum(const uint16_t* dy, int ld, int cout, int M, int mchunk,
                                                 float* part) {
-    const int n = threadIdx.x;
-    if (n >= cout) return;
+    __shared__ float sp[256 * 8];
+    const int g = cout / 8, R = 256 / g;
+    const int t = threadIdx.x, c8 = t % g, rg = t / g;
     const int mb = blockIdx.x * mchunk, me = min(M, mb + mchunk);
-    float s = 0.f;
-    for (int m = mb; m < me; m++) s += bf2f(dy[(int64_t)m * ld + n]);
-    part[(int64_t)blockIdx.x * cout + n] = s;
+    float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (rg < R)
+        for (int m = mb + rg; m < me; m += R) {
+            const uint4 v = *(const uint4*)(dy + (int64_t)m * ld + c8 * 8);
+            const uint16_t* pv = (const uint16_t*)&v;
+            for (int j = 0; j < 8; j++) acc[j] += bf2f(pv[j]);
+        }
+    for (int j = 0; j < 8; j++) sp[t * 8 + j] = acc[j];
+    __syncthreads();
+    if (t < cout) {
+        const int cg = t / 8, j = t % 8;
+        float s = 0.f;
+        for (int r = 0; r < R; r++) s += sp[(r * g + cg) * 8 + j];
+        part[(int64_t)blockIdx.x * cout + t] = s;
+    }
 }
 
 // dst[i] = sum_s src[s * len + i], s ascending
@@ -363,11 +427,15 @@ __global__ void k_relu_bwd(int64_t m, int c8n, const uint16_t* a, int lda, uint1
 // starts from g_in = 0 on solid voxels and "infinite" elsewhere.  One thread per line.
 constexpr float EDT_INF = 1e30f;
 
-__global__ void k_edt_pass(pvnet_shape s, int axis, const uint8_t* code, float* g) {
+// One thread per line; the block's lines are staged in LDS (row pitch n + 1: the threads' reads
+// of one column fall in distinct banks).
+__global__ __launch_bounds__(64) void k_edt_pass(pvnet_shape s, int axis, const uint8_t* code, float* g) {
+    extern __shared__ float sh[];
     const int64_t L = (int64_t)s.nb * s.d * s.h * s.w;
     const int n = axis == 0 ? s.w : axis == 1 ? s.h : s.d;
     const int64_t nlines = L / n;
     const int64_t line = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    float* in = sh + (int64_t)threadIdx.x * (n + 1);
     if (line >= nlines) return;
     int64_t base, stride;
     if (axis == 0) {
@@ -382,7 +450,6 @@ __global__ void k_edt_pass(pvnet_shape s, int axis, const uint8_t* code, float* 
         base = b * (int64_t)s.d * s.h * s.w + xy;
         stride = (int64_t)s.w * s.h;
     }
-    float in[PVNET_EDT_MAX];
     for (int j = 0; j < n; j++)
         in[j] = axis == 0 ? (code[base + j * stride] == 0 ? 0.f : EDT_INF) : g[base + j * stride];
     for (int i = 0; i < n; i++) {
@@ -550,13 +617,13 @@ WgradPlan plan_wgrad(const pvnet_shape& s, int ntaps, int cin, int cout) {
     p.ktiles = (K + 63) / 64;
     p.ntiles = (cout + p.bn - 1) / p.bn;
     const int64_t want = 2048 / (p.ktiles * p.ntiles) + 1;
-    int64_t steps = M / 32;
+    int64_t steps = (M + 127) / 128;  // LDS fills of 128 voxels
     int64_t ns = std::min<int64_t>(std::max<int64_t>(want, 1), std::max<int64_t>(steps, 1));
     int64_t per = (steps + ns - 1) / ns;
-    p.mchunk = (int)(per * 32);
+    p.mchunk = (int)(per * 128);
     p.nsplit = (int)((M + p.mchunk - 1) / p.mchunk);
     p.ws_main = (int64_t)p.nsplit * cout * K;
-    p.cchunk = 2048;
+    p.cchunk = 8192;
     p.cblk = (int)((M + p.cchunk - 1) / p.cchunk);
     p.ws_total = p.ws_main + (int64_t)p.cblk * cout;
     return p;
@@ -633,12 +700,12 @@ int pvnet_conv_wgrad(const pvnet_wgrad* a, void* stream) {
                        (int64_t)K, (int64_t)K);
     NET_LAUNCH();
     if (a->db) {
-        if (a->cout > 256) return fail(1, "pvnet_conv_wgrad: bias gradient supports cout <= 256");
+        if (a->cout > 256 || a->cout % 8) return fail(1, "pvnet_conv_wgrad: bias gradient needs cout <= 256, a multiple of 8");
         float* part = a->ws + p.ws_main;
         hipLaunchKernelGGL(k_colsum, dim3(p.cblk), dim3(256), 0, st, a->dy, a->lddy, a->cout, (int)M, p.cchunk, part);
         NET_LAUNCH();
         hipLaunchKernelGGL(k_sum_splits, dim3(1), dim3(256), 0, st, part, p.cblk, (int64_t)a->cout, a->db,
-                           (int64_t)a->cout, (int64_t)a->cout);
+                           (int64_t)a->cout, (int64_t)a->cout);  // cblk <= a few hundred partial rows
         NET_LAUNCH();
     }
     return 0;
@@ -710,8 +777,10 @@ int pvnet_wall_distance(const pvnet_shape* s, const uint8_t* code, float* dist, 
     const int len[3] = {s->w, s->h, s->d};
     for (int axis = 0; axis < 3; axis++) {
         const int64_t lines = n / len[axis];
-        hipLaunchKernelGGL(k_edt_pass, dim3(blocks_for(lines, 64)), dim3(64), 0, (hipStream_t)stream, *s, axis, code,
-                           dist);
+        const int threads = len[axis] <= 128 ? 64 : len[axis] <= 256 ? 32 : 16;  // LDS <= 33 KB
+        const size_t lds = (size_t)threads * (len[axis] + 1) * sizeof(float);
+        hipLaunchKernelGGL(k_edt_pass, dim3(blocks_for(lines, threads)), dim3(threads), lds, (hipStream_t)stream, *s,
+                           axis, code, dist);
         NET_LAUNCH();
     }
     return 0;

@@ -256,6 +256,70 @@ __global__ __launch_bounds__(256) void k_copy(const double2* __restrict__ src, d
 
 }  // namespace
 
+// coarse-field expansion for starts from a network prediction
+__device__ __forceinline__ float coarse_coord(int64_t i, int64_t n, int c, int& i0, int& i1) {
+    float x = ((float)i + 0.5f) * (float)c / (float)n - 0.5f;
+    x = fminf(fmaxf(x, 0.f), (float)(c - 1));
+    i0 = (int)floorf(x);
+    i1 = min(i0 + 1, c - 1);
+    return x - (float)i0;
+}
+
+__global__ void k_expand_coarse(const float* rho_c, const float* u_c, int cx, int cy, int cz, int64_t nx,
+                                int64_t ny, int64_t nz, int64_t nxp, int64_t nyp, int64_t ns, double umax,
+                                double rlo, double rhi, double* out, unsigned long long* nonfin) {
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= ns) return;
+    const int64_t x = n % nxp, pr = n / nxp, y = pr % nyp, z = pr / nyp;
+    double r = 1.0, u0 = 0.0, u1 = 0.0, u2 = 0.0;
+    if (x < nx && y < ny && z < nz) {
+        int x0, x1, y0, y1, z0, z1;
+        const float fx = coarse_coord(x, nx, cx, x0, x1);
+        const float fy = coarse_coord(y, ny, cy, y0, y1);
+        const float fz = coarse_coord(z, nz, cz, z0, z1);
+        const int64_t plane = (int64_t)cx * cy * cz;
+        float v[4];
+        for (int c = 0; c < 4; c++) {
+            const float* f = c == 0 ? rho_c : u_c + (c - 1) * plane;
+            auto at = [&](int zz, int yy, int xx) { return f[((int64_t)zz * cy + yy) * cx + xx]; };
+            const float a = at(z0, y0, x0) * (1.f - fx) + at(z0, y0, x1) * fx;
+            const float b = at(z0, y1, x0) * (1.f - fx) + at(z0, y1, x1) * fx;
+            const float cc = at(z1, y0, x0) * (1.f - fx) + at(z1, y0, x1) * fx;
+            const float d = at(z1, y1, x0) * (1.f - fx) + at(z1, y1, x1) * fx;
+            v[c] = (a * (1.f - fy) + b * fy) * (1.f - fz) + (cc * (1.f - fy) + d * fy) * fz;
+        }
+        if (isfinite(v[0]) && isfinite(v[1]) && isfinite(v[2]) && isfinite(v[3])) {
+            r = fmin(fmax((double)v[0], rlo), rhi);
+            u0 = v[1];
+            u1 = v[2];
+            u2 = v[3];
+            const double sp = sqrt(u0 * u0 + u1 * u1 + u2 * u2);
+            if (sp > umax) {
+                const double f = umax / sp;
+                u0 *= f;
+                u1 *= f;
+                u2 *= f;
+            }
+        } else {
+            atomicAdd(nonfin, 1ull);
+        }
+    }
+    out[n] = r;
+    out[ns + n] = u0;
+    out[2 * ns + n] = u1;
+    out[3 * ns + n] = u2;
+}
+
+hipError_t launch_expand_coarse(const float* rho_c, const float* u_c, int cx, int cy, int cz, int64_t nx,
+                                int64_t ny, int64_t nz, int64_t nxp, int64_t nyp, int64_t ns, double umax,
+                                double rlo, double rhi, double* out, unsigned long long* nonfin,
+                                hipStream_t s) {
+    const int64_t blocks = (ns + 255) / 256;
+    hipLaunchKernelGGL(k_expand_coarse, dim3((unsigned)blocks), dim3(256), 0, s, rho_c, u_c, cx, cy, cz, nx, ny,
+                       nz, nxp, nyp, ns, umax, rlo, rhi, out, nonfin);
+    return hipGetLastError();
+}
+
 hipError_t launch_copy(const void* src, void* dst, int64_t bytes, hipStream_t s) {
     const int64_t n = bytes / 16;
     hipLaunchKernelGGL(k_copy, dim3(8192), dim3(256), 0, s, static_cast<const double2*>(src),

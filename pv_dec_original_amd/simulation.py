@@ -162,6 +162,24 @@ class Simulation:
             self._chk(N.lib().pvlbm_init_fields(self.handle, N.dptr(r), N.dptr(v)))
         self.total_runtime = 0.0
 
+    def init_fields_coarse(self, rho_c, u_c, umax, rho_range):
+        """Restart from a coarse FP32 field held in device memory (torch tensors rho (cz, cy, cx)
+        and u (3, cz, cy, cx)), sampled trilinearly at every node with speeds clipped to umax and
+        densities into rho_range.  Returns the number of nodes whose sample was not finite
+        (started at rest)."""
+        if rho_c.dtype.itemsize != 4 or u_c.dtype.itemsize != 4 or not (rho_c.is_contiguous() and u_c.is_contiguous()):
+            raise ValueError("rho_c and u_c must be contiguous FP32 device tensors")
+        cz, cy, cx = rho_c.shape
+        if tuple(u_c.shape) != (3, cz, cy, cx):
+            raise ValueError("u_c must be (3,) + rho_c.shape")
+        dims = (C.c_int32 * 3)(cx, cy, cz)
+        nf = C.c_int64(0)
+        self._chk(N.lib().pvlbm_init_fields_coarse(self.handle, C.c_void_p(rho_c.data_ptr()),
+                                                   C.c_void_p(u_c.data_ptr()), dims, float(umax),
+                                                   float(rho_range[0]), float(rho_range[1]), C.byref(nf)))
+        self.total_runtime = 0.0
+        return nf.value
+
     def step(self, n, inlet_scale=None, timed=False):
         """n iterations of the loop body; inlet_scale: n values (None = 1.0)."""
         sc = None if inlet_scale is None else np.ascontiguousarray(inlet_scale, dtype=np.float64)

@@ -51,6 +51,8 @@ def main(argv=None):
     ap.add_argument("--levels", default="16,32,64,160")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--augment", type=int, default=1, help="random symmetry per minibatch")
+    ap.add_argument("--init", default=None, help="continue from these weights (npz of save_weights)")
+    ap.add_argument("--seed0", type=int, default=0, help="first training seed (later runs take new vessels)")
     ap.add_argument("--out", required=True)
     a = ap.parse_args(argv)
 
@@ -86,7 +88,7 @@ def main(argv=None):
         from concurrent.futures import ThreadPoolExecutor
         with ThreadPoolExecutor(max_workers=a.workers) as ex:
             futs = []
-            for seed in range(a.samples):
+            for seed in range(a.seed0, a.seed0 + a.samples):
                 if stop.is_set():
                     break
                 futs.append(ex.submit(W.solve_sample, seed, n))
@@ -100,8 +102,13 @@ def main(argv=None):
     prod = threading.Thread(target=produce, daemon=True)
     prod.start()
 
-    spec = UNetSpec(levels=tuple(int(v) for v in a.levels.split(",")))
-    net = UNet(spec, seed=a.seed)
+    if a.init:
+        levels, params = W.load_weights(a.init)
+        spec = UNetSpec(levels=levels)
+        net = UNet(spec, params)
+    else:
+        spec = UNetSpec(levels=tuple(int(v) for v in a.levels.split(",")))
+        net = UNet(spec, seed=a.seed)
     log(f"network: {spec.nparams} parameters, levels {spec.levels}")
 
     have = 0
@@ -178,8 +185,14 @@ def main(argv=None):
     torch.cuda.synchronize()
     vm, vr = W.evaluate(net, vc, vt, vp, a.batch)
     train_s = time.perf_counter() - (t_train0 or time.perf_counter())
+    prev = {}
+    if a.init:
+        with np.load(a.init, allow_pickle=False) as z:
+            prev = {k: float(z[k]) for k in ("total_steps", "total_samples") if k in z}
     meta = {"samples": have, "val_samples": a.val, "steps": step, "batch": a.batch, "val_mse": vm,
-            "val_rel_u": vr, "train_seconds": train_s, "seed": a.seed}
+            "val_rel_u": vr, "train_seconds": train_s, "seed": a.seed, "seed0": a.seed0,
+            "total_steps": prev.get("total_steps", 0) + step,
+            "total_samples": prev.get("total_samples", 0) + have}
     W.save_weights(os.path.join(a.out, "warmstart_unet.npz"), net, meta)
     rec = {"args": vars(a), "meta": meta, "history": hist, "statuses": statuses,
            "gen_done_at_s": gen_done_at, "gen_steps_mean": float(np.mean(gen_steps)) if gen_steps else None,

@@ -186,26 +186,27 @@ def coarse_params(u_mean, tau, s):
     return uc, tc
 
 
-def resample_code(code, cshape):
-    """Code on a coarser grid cshape by nearest sampling at the coarse voxel centres; the fluid
-    voxels of the first and last x planes are the inlet and outlet faces when the fine grid has
-    them there."""
-    if tuple(cshape) == tuple(code.shape):
-        return code.copy()
-    idx = [np.minimum(((np.arange(c) + 0.5) * n / c).astype(np.int64), n - 1) for c, n in zip(cshape, code.shape)]
-    c = code[np.ix_(*idx)].copy()
-    c[c > 1] = 1
+def coarse_code(kinds, cshape):
+    """Network input code on the grid cshape from the solver's node kinds: nearest sampling at the
+    coarse voxel centres; the fluid voxels of the first and last x planes are the inlet and outlet
+    faces when the solver grid has inlet and outlet nodes there.  On the solver grid itself this is
+    code_from_kinds."""
+    if tuple(cshape) == tuple(kinds.shape):
+        return code_from_kinds(kinds)
+    idx = [np.minimum(((np.arange(c) + 0.5) * n / c).astype(np.int64), n - 1) for c, n in zip(cshape, kinds.shape)]
+    c = (kinds[np.ix_(*idx)] != N.SOLID).astype(np.uint8)
     fl = c > 0
-    if (code[:, :, 0] == 2).any():
+    if (kinds[:, :, 0] == N.KINLET).any():
         c[:, :, 0][fl[:, :, 0]] = 2
-    if (code[:, :, -1] == 3).any():
+    if (kinds[:, :, -1] == N.KOUTLET).any():
         c[:, :, -1][fl[:, :, -1]] = 3
     return c
 
 
 def _trilinear_up(field, shape):
     """Values at the fine nodes of `shape` from values at the centres of the coarse voxels of
-    `field` (coarse voxel k spans fine coordinates [k, k+1) * n/c); clamped at the edges."""
+    `field` (coarse voxel k spans fine coordinates [k, k+1) * n/c); clamped at the edges.  The
+    host restatement of the device expansion (pvlbm_init_fields_coarse), used by the tests."""
     out = field
     for axis, n in enumerate(shape):
         nc = field.shape[axis]
@@ -227,20 +228,6 @@ class WarmStart:
             net = UNet(UNetSpec(levels=levels), params)
         self.net = net
 
-    def predict(self, code, u_mean, tau):
-        """(rho, u) on the grid of `code` (d, h, w), u as (3, d, h, w), numpy FP64."""
-        import torch
-        d, h, w = code.shape
-        cd = torch.from_numpy(np.ascontiguousarray(code)).to("cuda").reshape(1, d, h, w)
-        prm = torch.tensor([[u_mean, tau, float(np.count_nonzero(code == 2))]], dtype=torch.float32,
-                           device="cuda")
-        y = self.net.forward(cd, prm)
-        torch.cuda.synchronize()
-        y = y.view(d, h, w, -1).cpu().numpy().astype(np.float64)
-        u = np.moveaxis(y[..., 0:3], -1, 0) * u_mean
-        rho = 1.0 + y[..., 3] * u_mean
-        return rho, u
-
     def net_grid(self, shape):
         """The network's grid for a solver grid: the solver grid itself when it fits (at most
         NET_MAX voxels per axis, sizes divisible by the network's divisor), else a grid with the
@@ -253,26 +240,46 @@ class WarmStart:
         s = max(s, 1.0)
         return tuple(max(div, int(round(n / s / div)) * div) for n in shape), s
 
-    def fields(self, kinds, u_mean, tau):
-        """Initial (rho, u) on the solver grid of `kinds` (nz, ny, nx)."""
-        code = code_from_kinds(kinds)
+    def predict(self, code, u_mean, tau):
+        """(rho, u) on the grid of `code` (d, h, w) as FP32 device tensors (d, h, w) and
+        (3, d, h, w), denormalised with (u_mean, tau) as given to the network."""
+        import torch
+        d, h, w = code.shape
+        cd = torch.from_numpy(np.ascontiguousarray(code)).to("cuda").reshape(1, d, h, w)
+        prm = torch.tensor([[u_mean, tau, float(np.count_nonzero(code == 2))]], dtype=torch.float32,
+                           device="cuda")
+        y = self.net.forward(cd, prm).view(d, h, w, -1)
+        u = y[..., 0:3].permute(3, 0, 1, 2).contiguous() * u_mean
+        rho = 1.0 + y[..., 3].contiguous() * u_mean
+        return rho, u
+
+    def coarse_fields(self, kinds, u_mean, tau):
+        """The network's field for a solver grid, on the network grid: device tensors rho
+        (cz, cy, cx) and u (3, cz, cy, cx) for the solver's (u_mean, tau), rest on solid voxels,
+        plus the coarse code."""
         cshape, s = self.net_grid(kinds.shape)
-        cc = resample_code(code, cshape)
+        cc = coarse_code(kinds, cshape)
         uc, tc = coarse_params(u_mean, tau, s) if s > 1 else (u_mean, tau)
-        rho_c, u_c = self.predict(cc, uc, tc)
+        rho, u = self.predict(cc, uc, tc)
         # at equal Reynolds number u / u_mean is the same field, and rho - 1 (viscous drop
         # nu u L / R^2 or inertial u^2 alike) scales with u_mean^2 / u_c^2
-        u_c = u_c * (u_mean / uc)
-        rho_c = 1.0 + (rho_c - 1.0) * (u_mean / uc) ** 2
-        if cshape == tuple(kinds.shape):
-            rho, u = rho_c, u_c
-        else:
-            rho = _trilinear_up(rho_c, kinds.shape)
-            u = np.stack([_trilinear_up(u_c[k], kinds.shape) for k in range(3)])
-        fluid = kinds != N.SOLID
-        rho = np.where(fluid, rho, 1.0)
-        u = np.where(fluid[None], u, 0.0)
-        return rho, u
+        k = u_mean / uc
+        import torch
+        solid = torch.from_numpy(cc == 0).to("cuda")
+        rho = torch.where(solid, torch.ones_like(rho), 1.0 + (rho - 1.0) * (k * k))
+        u = torch.where(solid[None], torch.zeros_like(u), u * k)
+        return rho, u, cc
+
+    def apply(self, sim, sd, u_mean, tau):
+        """Start sim from the network's field (True) or leave it untouched when the field fails
+        field_ok (False)."""
+        kinds = sd.kinds()
+        rho, u, cc = self.coarse_fields(kinds, u_mean, tau)
+        fl = cc > 0
+        if not field_ok(rho.cpu().numpy(), u.cpu().numpy(), fl):
+            return False
+        nonfin = sim.init_fields_coarse(rho, u, UMAX, RHO_CLIP)
+        return nonfin == 0
 
 
 def field_ok(rho, u, fluid):
@@ -298,15 +305,11 @@ def start(sim, sd, warm, u_mean, tau):
     """Initialise sim from the network's field when it passes field_ok, else from rest.
     Returns 'warm' or 'rest'."""
     if warm is not None:
-        kinds = sd.kinds()
         try:
-            rho, u = warm.fields(kinds, u_mean, tau)
+            if warm.apply(sim, sd, u_mean, tau):
+                return "warm"
         except (ValueError, N.NativeError):
-            rho = u = None
-        if rho is not None and field_ok(rho, u, kinds != N.SOLID):
-            rho, u = clip_field(rho, u)
-            sim.init_fields(rho, u)
-            return "warm"
+            pass
     sim.init_fields()
     return "rest"
 

@@ -23,11 +23,18 @@ class _SD:
 
 
 class _Net:
-    def __init__(self, rho, u):
-        self.rho, self.u = rho, u
+    """Stands in for WarmStart.apply: the host check, then a start from the field."""
 
-    def fields(self, kinds, u_mean, tau):
-        return self.rho, self.u
+    def __init__(self, rho, u, fail=None):
+        self.rho, self.u, self.fail = rho, u, fail
+
+    def apply(self, sim, sd, u_mean, tau):
+        if self.fail:
+            raise self.fail
+        if not W.field_ok(self.rho, self.u, sd.kinds() != N.SOLID):
+            return False
+        sim.init_fields(*W.clip_field(self.rho, self.u))
+        return True
 
 
 def _kinds():
@@ -69,6 +76,9 @@ def test_field_checks_and_fallback_to_rest():
     assert np.array_equal(vc[:, 1, 1, 3], v[:, 1, 1, 3])
     sim = _Sim()
     assert W.start(sim, _SD(k), None, 0.03, 0.7) == "rest" and sim.calls == ["rest"]
+    sim = _Sim()
+    assert W.start(sim, _SD(k), _Net(rho, u, fail=N.NativeError("x")), 0.03, 0.7) == "rest"
+    assert sim.calls == ["rest"]
     # values on solid nodes do not matter
     v = u.copy()
     v[:, ~fl] = np.nan
@@ -88,16 +98,17 @@ def test_coarse_params_keep_reynolds_number():
     assert np.isclose(2 * 0.03 / (0.4 / 3), uc / ((tc - 0.5) / 3), rtol=1e-12)
 
 
-def test_resample_code_and_trilinear_upsampling():
-    code = np.zeros((8, 8, 12), dtype=np.uint8)
-    code[2:6, 2:6, :] = 1
-    code[2:6, 2:6, 0] = 2
-    code[2:6, 2:6, -1] = 3
-    c = W.resample_code(code, (4, 4, 6))
+def test_coarse_code_and_trilinear_upsampling():
+    kinds = np.zeros((8, 8, 12), dtype=np.uint8)
+    kinds[2:6, 2:6, :] = N.FLUID
+    kinds[2:6, 2:6, 0] = N.KINLET
+    kinds[2:6, 2:6, -1] = N.KOUTLET
+    kinds[2, 2:6, 3:9] = N.WALL
+    c = W.coarse_code(kinds, (4, 4, 6))
     assert c.shape == (4, 4, 6)
     assert (c[:, :, 0][c[:, :, 0] > 0] == 2).all() and (c[:, :, -1][c[:, :, -1] > 0] == 3).all()
     assert set(np.unique(c[:, :, 1:-1])) <= {0, 1} and (c[1:3, 1:3, 1:-1] == 1).all()
-    assert W.resample_code(code, code.shape).tolist() == code.tolist()
+    assert W.coarse_code(kinds, kinds.shape).tolist() == W.code_from_kinds(kinds).tolist()
     # trilinear interpolation reproduces a linear field between the coarse centres, for
     # integer and non-integer ratios alike
     for shape, cshape in [((8, 8, 12), (4, 4, 6)), ((10, 12, 16), (4, 5, 6))]:

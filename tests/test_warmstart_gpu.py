@@ -35,3 +35,33 @@ def test_start_from_fields_keeps_the_steady_state(W):
     w = sim.wss()[:, 3]
     err = np.linalg.norm(w - ref) / np.linalg.norm(ref)
     assert err < 2e-3, err
+
+
+def test_coarse_start_matches_host_expansion(W):
+    """pvlbm_init_fields_coarse against the host restatement of its sampling and clipping."""
+    import torch
+    from pv_dec_original_amd import _native as N
+    case, sd, sim, um, ta = W.vessel_setup(VAL_SEED0 + 2)
+    kinds = sd.kinds()
+    fl = kinds != N.SOLID
+    rng = np.random.default_rng(3)
+    cshape = (40, 48, 56)
+    rho_c = (1.0 + 0.05 * rng.standard_normal(cshape)).astype(np.float32)
+    u_c = (0.05 * rng.standard_normal((3,) + cshape)).astype(np.float32)
+    u_c[0, 20, 24, 28] = 2.0  # speeds around it are clipped
+    rho_c[10, 30, 30] = 3.0  # densities around it are clipped
+    rd, ud = torch.from_numpy(rho_c).cuda(), torch.from_numpy(u_c).cuda()
+    assert sim.init_fields_coarse(rd, ud, W.UMAX, W.RHO_CLIP) == 0
+    rho, u = sim.macros()
+    rr = W._trilinear_up(rho_c.astype(np.float64), kinds.shape)
+    uu = np.stack([W._trilinear_up(u_c[k].astype(np.float64), kinds.shape) for k in range(3)])
+    rr, uu = W.clip_field(rr, uu)
+    assert np.abs(rho[fl] - rr[fl]).max() < 1e-5
+    assert np.abs(u[:, fl] - uu[:, fl]).max() < 1e-5
+    assert np.sqrt((u[:, fl] ** 2).sum(axis=0)).max() <= W.UMAX * (1 + 1e-9)
+    # non-finite samples start at rest and are counted
+    rho_c[5, 5, 5] = np.nan
+    rd = torch.from_numpy(rho_c).cuda()
+    assert sim.init_fields_coarse(rd, ud, W.UMAX, W.RHO_CLIP) > 0
+    rho, u = sim.macros()
+    assert np.isfinite(rho[fl]).all() and np.isfinite(u[:, fl]).all()
