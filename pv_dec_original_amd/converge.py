@@ -23,10 +23,18 @@ class PeriodicRun:
     kernel_ms: float = 0.0
     start_s: float = 0.0
     start_mode: str = "rest"
-    converged_at: int = None  # period index (1-based) at which the criterion held
-    wall_at_converged_s: float = None
-    history: list = field(default_factory=list)  # per period: change and/or error vs reference
-    record: np.ndarray = None
+    history: list = field(default_factory=list)  # per period: wall, kernel time, change, error
+    records: list = field(default_factory=list)  # per period: WSS record (samples x nodes)
+
+    def first(self, key, tol, min_period=1):
+        """First period (1-based) whose `key` is at most tol, or None."""
+        for h in self.history:
+            if h["period"] >= min_period and key in h and h[key] <= tol:
+                return h["period"]
+        return None
+
+    def wall_at(self, period):
+        return None if period is None else self.history[period - 1]["wall_s"]
 
 
 def pulsatile_scales(it0, n, period, mean, amp):
@@ -34,12 +42,16 @@ def pulsatile_scales(it0, n, period, mean, amp):
     return mean + amp * np.sin(2.0 * np.pi * t / period)
 
 
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
 def run_periods(sim, label, period=4000, sample_every=200, mean=0.03, amp=0.02, max_periods=40,
-                rel_change=None, reference=None, tol=0.01, start=None, log=None, min_periods=2):
-    """Run whole inlet periods from the current state.  `start` (optional) is a callable that
-    initialises sim and returns its mode; its time counts.  Stops when the period-to-period change
-    is below rel_change (plain path) or when the record is within tol of reference (chosen
-    path)."""
+                stop_change=0.01, reference=None, start=None, log=None, min_periods=2):
+    """Run whole inlet periods from the state set by `start` (a callable returning the start mode;
+    its time counts) until the period-to-period change of the WSS record is below stop_change
+    (from period min_periods on) or max_periods.  With a reference record, each period's error
+    against it is kept too."""
     res = PeriodicRun(label)
     t0 = time.perf_counter()
     if start is not None:
@@ -67,21 +79,14 @@ def run_periods(sim, label, period=4000, sample_every=200, mean=0.03, amp=0.02, 
                 log(f"{label}: non-finite WSS in period {P}")
             break
         if prev is not None:
-            h["change"] = float(np.linalg.norm(rec - prev) / max(np.linalg.norm(rec), 1e-300))
+            h["change"] = rel(rec, prev)
         if reference is not None:
-            h["error"] = float(np.linalg.norm(rec - reference) / max(np.linalg.norm(reference), 1e-300))
+            h["error"] = rel(rec, reference)
         res.history.append(h)
+        res.records.append(rec)
         if log:
             log(f"{label}: period {P} " + " ".join(f"{k} {v:.4g}" for k, v in h.items() if k != "period"))
         prev = rec
-        done = False
-        if rel_change is not None and "change" in h and h["change"] < rel_change and P >= min_periods:
-            done = True
-        if reference is not None and h["error"] <= tol:
-            done = True
-        if done:
-            res.converged_at = P
-            res.wall_at_converged_s = res.wall_s
+        if "change" in h and h["change"] < stop_change and P >= min_periods:
             break
-    res.record = prev
     return res

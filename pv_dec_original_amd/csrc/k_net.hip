@@ -73,11 +73,12 @@ __device__ __forceinline__ void tap_offset(int ntaps, int tap, int& dz, int& dy,
 }
 
 // ---------------------------------------------------------------------------------------------
-// forward / input-gradient convolution: block tile 128 voxels x BN channels, 4 waves of 32 x BN
+// forward / input-gradient convolution: block tile 4*WM voxels x BN channels, 4 waves of WM x BN
 
-template <int BN>
+template <int BN, int WM>
 __global__ __launch_bounds__(256) void k_conv_fwd(pvnet_conv a, int M, int K) {
-    constexpr int BM = 128;
+    constexpr int BM = 4 * WM;
+    constexpr int NA = BM / 64;  // A rows loaded per thread and step
     __shared__ __attribute__((aligned(16))) uint16_t As[BM * LP];
     __shared__ __attribute__((aligned(16))) uint16_t Bs[BN * LP];
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
@@ -85,11 +86,12 @@ __global__ __launch_bounds__(256) void k_conv_fwd(pvnet_conv a, int M, int K) {
     const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
     const int d = a.s.d, h = a.s.h, w = a.s.w;
 
-    // load roles: A rows r and r + 64, 8-wide k chunk q; B row bn (if < BN), same chunk
+    // load roles: A rows r + 64 i, 8-wide k chunk q; B row bn (if < BN), same chunk
     const int q = t & 3, r = t >> 2, bn = t >> 2;
-    int zz[2], yy[2], xx[2];
-    bool ok[2];
-    for (int i = 0; i < 2; i++) {
+    int zz[NA], yy[NA], xx[NA];
+    bool ok[NA];
+    #pragma unroll
+    for (int i = 0; i < NA; i++) {
         int m = m0 + r + 64 * i;
         ok[i] = m < M;
         int mm = ok[i] ? m : 0;
@@ -100,7 +102,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd(pvnet_conv a, int M, int K) {
     }
     const bool brow = bn < BN && n0 + bn < a.cout;
     const int nsteps = a.kpad / BK;
-    uint4 ra[2], rb;
+    uint4 ra[NA], rb;
 
     auto load = [&](int s) {
         const int k = s * BK + q * 8;
@@ -110,7 +112,8 @@ __global__ __launch_bounds__(256) void k_conv_fwd(pvnet_conv a, int M, int K) {
         int dz, dy, dx;
         tap_offset(a.ntaps, tap, dz, dy, dx);
         const int64_t off = ((int64_t)dz * h + dy) * w + dx;
-        for (int i = 0; i < 2; i++) {
+        #pragma unroll
+        for (int i = 0; i < NA; i++) {
             const int z2 = zz[i] + dz, y2 = yy[i] + dy, x2 = xx[i] + dx;
             const bool in = ok[i] && kin && (unsigned)z2 < (unsigned)d && (unsigned)y2 < (unsigned)h &&
                             (unsigned)x2 < (unsigned)w;
@@ -121,33 +124,40 @@ __global__ __launch_bounds__(256) void k_conv_fwd(pvnet_conv a, int M, int K) {
         if (brow) rb = *(const uint4*)(a.w + (int64_t)(n0 + bn) * a.kpad + s * BK + q * 8);
     };
 
-    f32x4 acc[2][BN / 16];
-    for (int i = 0; i < 2; i++)
+    f32x4 acc[WM / 16][BN / 16];
+    #pragma unroll
+    for (int i = 0; i < WM / 16; i++)
         for (int j = 0; j < BN / 16; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
     load(0);
     for (int s = 0; s < nsteps; s++) {
-        for (int i = 0; i < 2; i++) *(uint4*)(As + (r + 64 * i) * LP + q * 8) = ra[i];
+        #pragma unroll
+        for (int i = 0; i < NA; i++) *(uint4*)(As + (r + 64 * i) * LP + q * 8) = ra[i];
         if (bn < BN) *(uint4*)(Bs + bn * LP + q * 8) = rb;
         __syncthreads();
         if (s + 1 < nsteps) load(s + 1);
-        bf16x8 af[2], bfr[BN / 16];
-        for (int i = 0; i < 2; i++) af[i] = *(const bf16x8*)(As + (wv * 32 + i * 16 + c16) * LP + g * 8);
+        bf16x8 af[WM / 16], bfr[BN / 16];
+        #pragma unroll
+        for (int i = 0; i < WM / 16; i++) af[i] = *(const bf16x8*)(As + (wv * WM + i * 16 + c16) * LP + g * 8);
+        #pragma unroll
         for (int j = 0; j < BN / 16; j++) bfr[j] = *(const bf16x8*)(Bs + (j * 16 + c16) * LP + g * 8);
-        for (int i = 0; i < 2; i++)
+        #pragma unroll
+        for (int i = 0; i < WM / 16; i++)
             for (int j = 0; j < BN / 16; j++)
                 acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
         __syncthreads();
     }
 
     const bool relu = a.flags & PVNET_RELU, f32out = a.flags & PVNET_F32OUT;
-    for (int i = 0; i < 2; i++)
+    #pragma unroll
+    for (int i = 0; i < WM / 16; i++)
         for (int j = 0; j < BN / 16; j++) {
             const int n = n0 + j * 16 + c16;
             if (n >= a.cout) continue;
             const float b = a.bias ? a.bias[n] : 0.f;
+#pragma unroll
             for (int e = 0; e < 4; e++) {
-                const int m = m0 + wv * 32 + i * 16 + g * 4 + e;
+                const int m = m0 + wv * WM + i * 16 + g * 4 + e;
                 if (m >= M) continue;
                 float v = acc[i][j][e] + b;
                 if (relu) v = v < 0.f ? 0.f : v;
@@ -656,13 +666,17 @@ int pvnet_conv_fwd(const pvnet_conv* c, void* stream) {
     if (M >= (int64_t)1 << 31) return fail(1, "pvnet_conv_fwd: too many voxels");
     hipStream_t st = (hipStream_t)stream;
     const int K = c->ntaps * c->cin;
+    // tiles: 256-voxel blocks for narrow outputs on large grids; 64-channel tiles only when the
+    // grid still has >= 2048 blocks (the K loop is long, so the device needs many blocks in flight)
     const int mt = blocks_for(M, 128);
-    if (c->cout <= 16) {
-        hipLaunchKernelGGL(k_conv_fwd<16>, dim3(mt, 1), dim3(256), 0, st, *c, (int)M, K);
-    } else if (c->cout <= 32 || c->cout % 64 != 0) {
-        hipLaunchKernelGGL(k_conv_fwd<32>, dim3(mt, (c->cout + 31) / 32), dim3(256), 0, st, *c, (int)M, K);
+    if (c->cout <= 16 && M >= (int64_t)256 * 2048) {
+        hipLaunchKernelGGL((k_conv_fwd<16, 64>), dim3(blocks_for(M, 256), 1), dim3(256), 0, st, *c, (int)M, K);
+    } else if (c->cout <= 16) {
+        hipLaunchKernelGGL((k_conv_fwd<16, 32>), dim3(mt, 1), dim3(256), 0, st, *c, (int)M, K);
+    } else if (c->cout % 64 == 0 && (int64_t)mt * (c->cout / 64) >= 2048) {
+        hipLaunchKernelGGL((k_conv_fwd<64, 32>), dim3(mt, c->cout / 64), dim3(256), 0, st, *c, (int)M, K);
     } else {
-        hipLaunchKernelGGL(k_conv_fwd<64>, dim3(mt, c->cout / 64), dim3(256), 0, st, *c, (int)M, K);
+        hipLaunchKernelGGL((k_conv_fwd<32, 32>), dim3(mt, (c->cout + 31) / 32), dim3(256), 0, st, *c, (int)M, K);
     }
     NET_LAUNCH();
     return 0;

@@ -131,6 +131,40 @@ def test_conv_fwd_and_dgrad_and_wgrad(T, case):
     assert np.allclose(_np(db), refdb, rtol=1e-5, atol=1e-5 * np.abs(refdb).max())
 
 
+def test_conv_fwd_large_grid_tiles(T):
+    """The 256-voxel tile used for narrow outputs on large grids, on sampled voxels against a direct
+    numpy sum over the 27 taps."""
+    from pv_dec_original_amd import _native as N
+    NL = N.net()
+    shape, cin, cout = (1, 64, 96, 96), 16, 16
+    M = O.nvox(shape)
+    x = _rbf((M, cin))
+    w = O.bf16_round(_rbf((cout, 27 * cin)) / np.sqrt(27 * cin))
+    b = RNG.standard_normal(cout).astype(np.float32).astype(np.float64)
+    xd, w32, bd = _dev(T, x), _dev(T, w, T.float32), _dev(T, b, T.float32)
+    kpad = (27 * cin + 31) // 32 * 32
+    wt = T.zeros((cout, kpad), dtype=T.bfloat16, device="cuda")
+    wd = T.zeros((cin, (27 * cout + 31) // 32 * 32), dtype=T.bfloat16, device="cuda")
+    N.net_check(NL.pvnet_pack_weights(_p(w32), 27, cin, cout, _p(wt), _p(wd), _st(T)))
+    yd = T.zeros((M, cout), dtype=T.bfloat16, device="cuda")
+    c = N.NetConv(N.NetShape(*shape), 27, cin, cout, kpad, _p(xd), cin, 1, _p(wt), _p(bd), _p(yd), cout, 0, None)
+    N.net_check(NL.pvnet_conv_fwd(C.byref(c), _st(T)))
+    T.cuda.synchronize()
+    got = _np(yd)
+    _, d, h, wd_ = shape
+    xs = x.reshape(d, h, wd_, cin)
+    ws = w.reshape(cout, 27, cin)
+    for m in RNG.integers(0, M, 300):
+        z, y, xx = m // (h * wd_), (m // wd_) % h, m % wd_
+        acc = b.copy()
+        for tap in range(27):
+            dz, dy, dx = tap // 9 - 1, (tap // 3) % 3 - 1, tap % 3 - 1
+            if 0 <= z + dz < d and 0 <= y + dy < h and 0 <= xx + dx < wd_:
+                acc += ws[:, tap, :] @ xs[z + dz, y + dy, xx + dx]
+        ref = np.maximum(acc, 0.0)
+        _close_bf16(got[m], ref, f"voxel {m}")
+
+
 def test_wgrad_deterministic(T):
     from pv_dec_original_amd import _native as N
     NL = N.net()
