@@ -175,13 +175,17 @@ int pvlbm_init_fields(pvlbm_handle h, const double* rho, const double* u);
 /* As pvlbm_init_fields from a coarse FP32 field in device memory: rho_c[cz][cy][cx] and
  * u_c[3][cz][cy][cx] (x fastest; cdims = {cx, cy, cz}) cover the grid, coarse voxel (i, j, k)
  * centred at fine coordinates ((i + 0.5) nx / cx - 0.5, ...); each node takes the trilinear
- * sample (clamped at the edges), its speed clipped to umax and its density into
- * [rho_lo, rho_hi].  Nodes whose sample is not finite start at rest; their number is returned in
- * *nonfinite.  No counterpart in the reference (it always starts from rest, Simulation.cpp:13-15):
- * the start from the learned warm-start network, without a host copy of the full-size field. */
+ * sample (clamped at the edges) with its density clipped into [rho_lo, rho_hi].  With
+ * flux_x > 0 the velocity of every plane x is then scaled (by a factor within [1/4, 4]) so that
+ * the flux of u_x over the plane's fluid nodes equals flux_x: for a grid whose inlet is the x-min
+ * face and whose outlets are all on the x-max face, mass conservation makes the inlet flux the
+ * flux of every plane.  Speeds are finally clipped to umax.  Nodes whose sample is not finite
+ * start at rest; their number is returned in *nonfinite.  No counterpart in the reference (it
+ * always starts from rest, Simulation.cpp:13-15): the start from the learned warm-start network,
+ * without a host copy of the full-size field. */
 int pvlbm_init_fields_coarse(pvlbm_handle h, const float* rho_c, const float* u_c,
                              const int32_t cdims[3], double umax, double rho_lo, double rho_hi,
-                             int64_t* nonfinite);
+                             double flux_x, int64_t* nonfinite);
 
 /* Every how many iterations WSS and its accumulators are evaluated (reference: every iteration,
  * Simulation.cpp:585-586).  Default 1. */

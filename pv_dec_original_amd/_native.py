@@ -89,7 +89,7 @@ PVLBM_API = [
     ("pvlbm_set_inlet_profile", C.c_int, [C.c_void_p, _d]),
     ("pvlbm_init_fields", C.c_int, [C.c_void_p, _d, _d]),
     ("pvlbm_init_fields_coarse", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_int32),
-                                           C.c_double, C.c_double, C.c_double, _i64]),
+                                           C.c_double, C.c_double, C.c_double, C.c_double, _i64]),
     ("pvlbm_set_wss_every", C.c_int, [C.c_void_p, C.c_int]),
     ("pvlbm_step", C.c_int, [C.c_void_p, C.c_int, _d]),
     ("pvlbm_step_timed", C.c_int, [C.c_void_p, C.c_int, _d, _d]),

@@ -790,7 +790,7 @@ extern "C" int pvlbm_init_fields(pvlbm_handle h, const double* rho, const double
 
 extern "C" int pvlbm_init_fields_coarse(pvlbm_handle h, const float* rho_c, const float* u_c,
                                         const int32_t cdims[3], double umax, double rho_lo, double rho_hi,
-                                        int64_t* nonfinite) {
+                                        double flux_x, int64_t* nonfinite) {
     if (!h) return PVLBM_EINVAL;
     return guarded(h, [&] {
         if (!rho_c || !u_c || !cdims) fail(PVLBM_EINVAL, "coarse fields and their dims are required");
@@ -800,11 +800,12 @@ extern "C" int pvlbm_init_fields_coarse(pvlbm_handle h, const float* rho_c, cons
         double* d_in = nullptr;
         unsigned long long* d_cnt = nullptr;
         try {
-            d_in = dalloc<double>((size_t)(4 * h->ns));
+            d_in = dalloc<double>((size_t)(4 * h->ns + h->nx));
             d_cnt = dalloc<unsigned long long>(1);
             HIPCHK(hipMemsetAsync(d_cnt, 0, sizeof(unsigned long long), h->stream));
             HIPCHK(launch_expand_coarse(rho_c, u_c, cdims[0], cdims[1], cdims[2], h->nx, h->ny, h->nz, h->nxp,
-                                        h->nyp, h->ns, umax, rho_lo, rho_hi, d_in, d_cnt, h->stream));
+                                        h->nyp, h->ns, umax, rho_lo, rho_hi, h->d_kind, flux_x,
+                                        d_in + 4 * h->ns, d_in, d_cnt, h->stream));
             h->reset_state();
             h->launch(F_INIT | F_MACRO | F_REDUCE, 1.0, d_in, d_in + h->ns);
             unsigned long long cnt = 0;

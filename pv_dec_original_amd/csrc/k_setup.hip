@@ -266,8 +266,8 @@ __device__ __forceinline__ float coarse_coord(int64_t i, int64_t n, int c, int& 
 }
 
 __global__ void k_expand_coarse(const float* rho_c, const float* u_c, int cx, int cy, int cz, int64_t nx,
-                                int64_t ny, int64_t nz, int64_t nxp, int64_t nyp, int64_t ns, double umax,
-                                double rlo, double rhi, double* out, unsigned long long* nonfin) {
+                                int64_t ny, int64_t nz, int64_t nxp, int64_t nyp, int64_t ns, double rlo,
+                                double rhi, double* out, unsigned long long* nonfin) {
     const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= ns) return;
     const int64_t x = n % nxp, pr = n / nxp, y = pr % nyp, z = pr / nyp;
@@ -293,13 +293,6 @@ __global__ void k_expand_coarse(const float* rho_c, const float* u_c, int cx, in
             u0 = v[1];
             u1 = v[2];
             u2 = v[3];
-            const double sp = sqrt(u0 * u0 + u1 * u1 + u2 * u2);
-            if (sp > umax) {
-                const double f = umax / sp;
-                u0 *= f;
-                u1 *= f;
-                u2 *= f;
-            }
         } else {
             atomicAdd(nonfin, 1ull);
         }
@@ -310,13 +303,63 @@ __global__ void k_expand_coarse(const float* rho_c, const float* u_c, int cx, in
     out[3 * ns + n] = u2;
 }
 
+// flux of u_x through each plane x over the fluid nodes: one block per plane, fixed-order sums
+__global__ __launch_bounds__(256) void k_slab_flux(const double* ux, const uint8_t* kind, int64_t ny, int64_t nz,
+                                                   int64_t nxp, int64_t nyp, double* flux) {
+    __shared__ double sh[256];
+    const int64_t x = blockIdx.x;
+    double sum = 0.0;
+    for (int64_t r = threadIdx.x; r < ny * nz; r += 256) {
+        const int64_t n = x + nxp * ((r % ny) + nyp * (r / ny));
+        if (kind[n] != 0) sum += ux[n];
+    }
+    sh[threadIdx.x] = sum;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) flux[x] = sh[0];
+}
+
+// velocity of plane x scaled by target / flux[x] (limited to [1/4, 4]; planes without flux keep
+// theirs), then speeds clipped to umax
+__global__ void k_scale_clip(double* out, int64_t ns, int64_t nx, int64_t nxp, const double* flux, double target,
+                             double umax) {
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= ns) return;
+    const int64_t x = n % nxp;
+    double f = 1.0;
+    if (flux && target > 0.0 && x < nx && flux[x] > 1e-12 * target) f = fmin(fmax(target / flux[x], 0.25), 4.0);
+    double u0 = out[ns + n] * f, u1 = out[2 * ns + n] * f, u2 = out[3 * ns + n] * f;
+    const double sp = sqrt(u0 * u0 + u1 * u1 + u2 * u2);
+    if (sp > umax) {
+        const double c = umax / sp;
+        u0 *= c;
+        u1 *= c;
+        u2 *= c;
+    }
+    out[ns + n] = u0;
+    out[2 * ns + n] = u1;
+    out[3 * ns + n] = u2;
+}
+
 hipError_t launch_expand_coarse(const float* rho_c, const float* u_c, int cx, int cy, int cz, int64_t nx,
                                 int64_t ny, int64_t nz, int64_t nxp, int64_t nyp, int64_t ns, double umax,
-                                double rlo, double rhi, double* out, unsigned long long* nonfin,
-                                hipStream_t s) {
+                                double rlo, double rhi, const uint8_t* kind, double flux_x, double* flux,
+                                double* out, unsigned long long* nonfin, hipStream_t s) {
     const int64_t blocks = (ns + 255) / 256;
     hipLaunchKernelGGL(k_expand_coarse, dim3((unsigned)blocks), dim3(256), 0, s, rho_c, u_c, cx, cy, cz, nx, ny,
-                       nz, nxp, nyp, ns, umax, rlo, rhi, out, nonfin);
+                       nz, nxp, nyp, ns, rlo, rhi, out, nonfin);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (flux_x > 0.0) {
+        hipLaunchKernelGGL(k_slab_flux, dim3((unsigned)nx), dim3(256), 0, s, out + ns, kind, ny, nz, nxp, nyp, flux);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_scale_clip, dim3((unsigned)blocks), dim3(256), 0, s, out, ns, nx, nxp,
+                       flux_x > 0.0 ? flux : nullptr, flux_x, umax);
     return hipGetLastError();
 }
 

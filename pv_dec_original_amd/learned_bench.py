@@ -45,12 +45,15 @@ def run(weights=DEFAULT_WEIGHTS, box=(1024, 512, 512), tau=0.6, period=4000, sam
     log(f"aneurysm geometry: {len(case.geometry.globalTriangles)} triangles, {time.perf_counter() - t0:.1f} s")
     warm = W.WarmStart(weights)
 
+    flux = {}
+
     def make(storage):
         t = time.perf_counter()
         sd = SubDomain(case.geometry, case.faces, storage=storage, tau=tau, dt=8e-6)
         sim = Simulation(sd, tau, 1.0, 8e-6)
         prof = inlet_disk_profile(case.geometry, sd, 1.0)
         sim.set_inlet_profile(prof)
+        flux["q"] = mean * float(prof.sum())  # the mean inlet flux
         log(f"{storage} subdomain: {sd.counts()} in {time.perf_counter() - t:.1f} s")
         return sd, sim
 
@@ -67,7 +70,7 @@ def run(weights=DEFAULT_WEIGHTS, box=(1024, 512, 512), tau=0.6, period=4000, sam
         return lambda: (sim.init_fields(), "rest")[1]
 
     def learned(sim, sd):
-        return lambda: W.start(sim, sd, warm, mean, tau)
+        return lambda: W.start(sim, sd, warm, mean, tau, flux["q"])
 
     # the plain path, continued past its own convergence to a tighter reference record
     plain = CV.run_periods(sim64, "fp64-rest", period, sample_every, mean, amp, max_periods,

@@ -162,11 +162,12 @@ class Simulation:
             self._chk(N.lib().pvlbm_init_fields(self.handle, N.dptr(r), N.dptr(v)))
         self.total_runtime = 0.0
 
-    def init_fields_coarse(self, rho_c, u_c, umax, rho_range):
+    def init_fields_coarse(self, rho_c, u_c, umax, rho_range, flux_x=0.0):
         """Restart from a coarse FP32 field held in device memory (torch tensors rho (cz, cy, cx)
-        and u (3, cz, cy, cx)), sampled trilinearly at every node with speeds clipped to umax and
-        densities into rho_range.  Returns the number of nodes whose sample was not finite
-        (started at rest)."""
+        and u (3, cz, cy, cx)), sampled trilinearly at every node, densities clipped into
+        rho_range, every x-plane's flux scaled to flux_x when flux_x > 0 (inlet on x-min, outlets
+        on x-max), speeds clipped to umax.  Returns the number of nodes whose sample was not
+        finite (started at rest)."""
         if rho_c.dtype.itemsize != 4 or u_c.dtype.itemsize != 4 or not (rho_c.is_contiguous() and u_c.is_contiguous()):
             raise ValueError("rho_c and u_c must be contiguous FP32 device tensors")
         cz, cy, cx = rho_c.shape
@@ -176,7 +177,8 @@ class Simulation:
         nf = C.c_int64(0)
         self._chk(N.lib().pvlbm_init_fields_coarse(self.handle, C.c_void_p(rho_c.data_ptr()),
                                                    C.c_void_p(u_c.data_ptr()), dims, float(umax),
-                                                   float(rho_range[0]), float(rho_range[1]), C.byref(nf)))
+                                                   float(rho_range[0]), float(rho_range[1]), float(flux_x),
+                                                   C.byref(nf)))
         self.total_runtime = 0.0
         return nf.value
 

@@ -13,8 +13,9 @@ solver grid resampled by a factor s when larger), with (u_mean, tau) mapped to t
 equal Reynolds number; its output is upsampled trilinearly to the solver grid and
 denormalised with the solver's u_mean.  A field that is not finite, or far out of range (a speed
 above REJECT_U, a density outside REJECT_RHO), is rejected and the solver starts from rest, the
-reference start; otherwise speeds are clipped to UMAX and densities into RHO_CLIP before
-pvlbm_init_fields sets f = f_eq(rho, u).
+reference start; otherwise the field is expanded on the device, every x-plane's flux is scaled to
+the inlet flux (mass conservation, for inlets on x-min and outlets on x-max), speeds are clipped to
+UMAX and densities into RHO_CLIP, and f = f_eq(rho, u) (pvlbm_init_fields_coarse).
 """
 import time
 from concurrent.futures import ThreadPoolExecutor
@@ -104,6 +105,7 @@ def vessel_setup(seed, n=96, q=19, storage="fp64", u_mean=None, tau=None):
     prof = inlet_disk_profile(case.geometry, sd, um)
     sim.set_inlet_profile(prof)
     sim.set_wss_every(1 << 30)
+    sim.inlet_flux = float(prof.sum())
     return case, sd, sim, um, ta
 
 
@@ -270,15 +272,22 @@ class WarmStart:
         u = torch.where(solid[None], torch.zeros_like(u), u * k)
         return rho, u, cc
 
-    def apply(self, sim, sd, u_mean, tau):
+    def apply(self, sim, sd, u_mean, tau, flux=None):
         """Start sim from the network's field (True) or leave it untouched when the field fails
-        field_ok (False)."""
+        field_ok (False).  When the grid's inlet nodes all lie on the x-min plane and its outlet
+        nodes on the x-max plane, the start enforces mass conservation: every x-plane carries the
+        inlet flux `flux` (default u_mean times the number of inlet nodes)."""
         kinds = sd.kinds()
         rho, u, cc = self.coarse_fields(kinds, u_mean, tau)
         fl = cc > 0
         if not field_ok(rho.cpu().numpy(), u.cpu().numpy(), fl):
             return False
-        nonfin = sim.init_fields_coarse(rho, u, UMAX, RHO_CLIP)
+        inl, outl = kinds == N.KINLET, kinds == N.KOUTLET
+        xfaces = inl.any() and outl.any() and not inl[:, :, 1:].any() and not outl[:, :, :-1].any()
+        q = 0.0
+        if xfaces:
+            q = float(u_mean * np.count_nonzero(inl[:, :, 0]) if flux is None else flux)
+        nonfin = sim.init_fields_coarse(rho, u, UMAX, RHO_CLIP, q)
         return nonfin == 0
 
 
@@ -301,12 +310,13 @@ def clip_field(rho, u):
     return np.clip(rho, *RHO_CLIP), u * f[None]
 
 
-def start(sim, sd, warm, u_mean, tau):
+def start(sim, sd, warm, u_mean, tau, flux=None):
     """Initialise sim from the network's field when it passes field_ok, else from rest.
-    Returns 'warm' or 'rest'."""
+    Returns 'warm' or 'rest'.  flux: the inlet flux (sum of the inlet velocities), see
+    WarmStart.apply."""
     if warm is not None:
         try:
-            if warm.apply(sim, sd, u_mean, tau):
+            if warm.apply(sim, sd, u_mean, tau, flux):
                 return "warm"
         except (ValueError, N.NativeError):
             pass

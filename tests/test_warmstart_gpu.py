@@ -65,3 +65,25 @@ def test_coarse_start_matches_host_expansion(W):
     assert sim.init_fields_coarse(rd, ud, W.UMAX, W.RHO_CLIP) > 0
     rho, u = sim.macros()
     assert np.isfinite(rho[fl]).all() and np.isfinite(u[:, fl]).all()
+
+
+def test_coarse_start_enforces_plane_flux(W):
+    """With a target flux every x-plane of the started field carries it (inlet on x-min, outlets
+    on x-max: mass conservation), before the solver takes a step."""
+    import torch
+    from pv_dec_original_amd import _native as N
+    case, sd, sim, um, ta = W.vessel_setup(VAL_SEED0 + 3)
+    kinds = sd.kinds()
+    fl = kinds != N.SOLID
+    rng = np.random.default_rng(4)
+    cshape = (48, 48, 48)
+    rho_c = np.ones(cshape, dtype=np.float32)
+    u_c = (0.004 * rng.random((3,) + cshape)).astype(np.float32)
+    u_c[0] += 0.01
+    q = 0.02 * np.count_nonzero(kinds[:, :, 0] == N.KINLET)
+    rd, ud = torch.from_numpy(rho_c).cuda(), torch.from_numpy(u_c).cuda()
+    assert sim.init_fields_coarse(rd, ud, W.UMAX, W.RHO_CLIP, q) == 0
+    rho, u = sim.macros()
+    flux = np.array([u[0][:, :, x][fl[:, :, x]].sum() for x in range(kinds.shape[2])])
+    has = np.array([fl[:, :, x].any() for x in range(kinds.shape[2])])
+    assert np.allclose(flux[has], q, rtol=1e-9)
