@@ -87,3 +87,27 @@ def test_coarse_start_enforces_plane_flux(W):
     flux = np.array([u[0][:, :, x][fl[:, :, x]].sum() for x in range(kinds.shape[2])])
     has = np.array([fl[:, :, x].any() for x in range(kinds.shape[2])])
     assert np.allclose(flux[has], q, rtol=1e-9)
+
+
+def test_trained_network_shortens_the_run_to_converged_wss(W):
+    """The committed network against the same solver without it: iterations until the WSS of a
+    validation vessel (seeds never trained on) is within 1% (L2) of its converged value, from
+    rest and from the warm start, FP64 D3Q19, constant inlet."""
+    from pv_dec_original_amd import _native as N
+    warm = W.WarmStart(os.path.join(os.path.dirname(W.__file__), "weights", "warmstart_unet.npz"))
+    rest_steps, warm_steps = [], []
+    for seed in [VAL_SEED0 + i for i in (1, 2, 3, 4, 5, 6, 8)]:
+        case, sd, sim, um, ta = W.vessel_setup(seed)
+        fluid = sd.kinds() != N.SOLID
+        ref, _, status = W.reference_wss(sim, fluid)
+        assert status == "converged", seed
+        sim.init_fields()
+        s_rest, _, _ = W.steps_to_wss(sim, ref, ramp=0, timed=False)
+        mode = W.start(sim, sd, warm, um, ta, sim.inlet_flux)
+        s_warm, _, _ = W.steps_to_wss(sim, ref, ramp=0, timed=False)
+        assert s_rest and s_warm, (seed, s_rest, s_warm)
+        rest_steps.append(s_rest)
+        warm_steps.append(s_warm if mode == "warm" else s_rest)
+    ratio = sum(warm_steps) / sum(rest_steps)
+    print(f"steps to 1% WSS: rest {rest_steps} warm {warm_steps} ratio {ratio:.3f}")
+    assert ratio <= 0.8, (rest_steps, warm_steps)
