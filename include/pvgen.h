@@ -51,6 +51,9 @@ void pvgen_mesh_free(pvgen_mesh* m);
 /* Write triangles as ASCII STL with 17 significant digits (round-trips every double). */
 int pvgen_write_stl_ascii(const char* path, const double* tri, int64_t ntri, const char* name);
 
+/* Write triangles as binary STL (single precision, the format's own; name in the header). */
+int pvgen_write_stl_binary(const char* path, const double* tri, int64_t ntri, const char* name);
+
 /* Signed distance-like field of the last shape family on points (negative inside), for tests:
  * evaluates the same field the mesher contoured. shape: 1 = Y-bifurcation with the parameters
  * of pvgen_ybifurcation packed as {rp, rd, angle_deg, xb, y0, z0, xcap0, xcap1}. */

@@ -113,6 +113,12 @@ int pvlbm_device_count(int* count);
  * silently, as the reference does.  Replaces Reader::readSTLASCII (GeometryReader.cpp:33-117). */
 int pvlbm_read_stl_ascii(const char* path, double* tri, int64_t capacity, int64_t* ntri);
 
+/* Parse a binary STL file (80-byte header, uint32 count, 50-byte facets); facet normals are
+ * normalised in single precision as the reference does (a zero normal becomes NaN).  A file
+ * shorter than its facet count is an error (PVLBM_EIO), where the reference would read past the
+ * end.  Replaces Reader::readSTLBinary (GeometryReader.cpp:119-176). */
+int pvlbm_read_stl_binary(const char* path, double* tri, int64_t capacity, int64_t* ntri);
+
 /* Bounding box {min_x,max_x,min_y,max_y,min_z,max_z} by the reference rule: x over all three
  * vertices, y and z over vertices 1 and 2 only.  Replaces the bbox part of
  * Geometry::GetSubBoundingBox (ComplexGeometry.cpp:142-154). */

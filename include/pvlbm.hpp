@@ -114,6 +114,13 @@ public:
         check(pvlbm_count(h_, which, &n), h_);
         return n;
     }
+    // node kind per grid point (PVLBM_SOLID .. PVLBM_WALL), x fastest
+    std::vector<uint8_t> kinds() const {
+        const size_t n = (size_t)geo_->grid.dims[0] * geo_->grid.dims[1] * geo_->grid.dims[2];
+        std::vector<uint8_t> k(n);
+        check(pvlbm_get_kinds(h_, k.data()), h_);
+        return k;
+    }
     std::vector<int64_t> InletBoundaryNode_() const { return list(PVLBM_LIST_INLET); }
     std::vector<int64_t> WallBoundaryNode_() const { return list(PVLBM_LIST_WALL); }
     std::vector<int64_t> nextToTheWall() const { return list(PVLBM_LIST_NTW); }
@@ -169,6 +176,14 @@ public:
     }
 
     /* x, y, z, WSS, OSI, TAWSS per next-to-wall node (OutputFile.cpp:884-894) */
+    // (rho, u) of every grid point, rho [nz][ny][nx] then u as three such planes (x, y, z)
+    void macros(std::vector<double>& rho, std::vector<double>& u) const {
+        const Geometry& g = sd_.Geometry_();
+        const size_t n = (size_t)g.grid.dims[0] * g.grid.dims[1] * g.grid.dims[2];
+        rho.resize(n);
+        u.resize(3 * n);
+        check(pvlbm_get_macros(h_, rho.data(), u.data()), h_);
+    }
     std::vector<double> surfaceAttributes() const {
         std::vector<double> out((size_t)(6 * sd_.count(PVLBM_LIST_NTW)));
         if (!out.empty()) check(pvlbm_get_wss(h_, out.data()), h_);

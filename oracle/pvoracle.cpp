@@ -324,6 +324,41 @@ int set_threads(int threads) {
 // Geometry
 // =============================================================================================
 
+extern "C" int64_t orc_read_stl_binary(const char* path, double* tri, int64_t capacity) {
+    // Reader::readSTLBinary (GeometryReader.cpp:119-176), stopping at the end of the file
+    std::ifstream file(path, std::ios::in | std::ios::binary);
+    if (!file) return -1;
+    char header[80];
+    file.read(header, 80);
+    uint32_t num_triangles = 0;
+    file.read(reinterpret_cast<char*>(&num_triangles), sizeof(num_triangles));
+    int64_t count = 0;
+    for (uint32_t i = 0; i < num_triangles; i++) {
+        float nx, ny, nz;
+        file.read(reinterpret_cast<char*>(&nx), sizeof(float));
+        file.read(reinterpret_cast<char*>(&ny), sizeof(float));
+        file.read(reinterpret_cast<char*>(&nz), sizeof(float));
+        float norm = std::sqrt(nx * nx + ny * ny + nz * nz);
+        nx /= norm;
+        ny /= norm;
+        nz /= norm;
+        float v[9];
+        file.read(reinterpret_cast<char*>(v), sizeof v);
+        uint16_t attr_byte_count;
+        file.read(reinterpret_cast<char*>(&attr_byte_count), sizeof(attr_byte_count));
+        if (!file) break;
+        if (tri && count < capacity) {
+            double* T = tri + 12 * count;
+            for (int k = 0; k < 9; k++) T[k] = v[k];
+            T[9] = nx;
+            T[10] = ny;
+            T[11] = nz;
+        }
+        count++;
+    }
+    return count;
+}
+
 extern "C" int64_t orc_read_stl_ascii(const char* path, double* tri, int64_t capacity) {
     // Reader::readSTLASCII (GeometryReader.cpp:33-117): facet normal / outer loop / 3 vertex /
     // endloop / endfacet; a facet whose lines do not parse is skipped.

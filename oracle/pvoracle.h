@@ -22,6 +22,9 @@ extern "C" {
 /* Reader::readSTLASCII (GeometryReader.cpp:33-117). */
 int64_t orc_read_stl_ascii(const char* path, double* tri, int64_t capacity);
 
+/* Reader::readSTLBinary (GeometryReader.cpp:119-176); -1 when the file cannot be opened. */
+int64_t orc_read_stl_binary(const char* path, double* tri, int64_t capacity);
+
 /* bbox rule (ComplexGeometry.cpp:142-154) and grid dims (:163-165). */
 void orc_bbox(const double* tri, int64_t ntri, double bb[6]);
 void orc_grid_dims(const double bb[6], double h, int32_t dims[3]);

@@ -76,6 +76,7 @@ PVLBM_API = [
     ("pvlbm_last_error", C.c_char_p, [C.c_void_p]),
     ("pvlbm_device_count", C.c_int, [C.POINTER(C.c_int)]),
     ("pvlbm_read_stl_ascii", C.c_int, [C.c_char_p, _d, C.c_int64, _i64]),
+    ("pvlbm_read_stl_binary", C.c_int, [C.c_char_p, _d, C.c_int64, _i64]),
     ("pvlbm_bbox", C.c_int, [_d, C.c_int64, _d]),
     ("pvlbm_grid_from_bbox", C.c_int, [_d, C.c_double, C.POINTER(Grid)]),
     ("pvlbm_voxelize", C.c_int, [_d, C.c_int64, C.c_double, C.POINTER(Grid), C.POINTER(Grid), _u8]),
@@ -140,6 +141,7 @@ PVGEN_API = [
     ("pvgen_mesh_copy", None, [C.c_void_p, _d]),
     ("pvgen_mesh_free", None, [C.c_void_p]),
     ("pvgen_write_stl_ascii", C.c_int, [C.c_char_p, _d, C.c_int64, C.c_char_p]),
+    ("pvgen_write_stl_binary", C.c_int, [C.c_char_p, _d, C.c_int64, C.c_char_p]),
     ("pvgen_field", C.c_int, [C.c_int, _d, C.c_int64, _d, _d]),
 ]
 

@@ -305,6 +305,40 @@ extern "C" int pvlbm_read_stl_ascii(const char* path, double* tri, int64_t capac
     });
 }
 
+extern "C" int pvlbm_read_stl_binary(const char* path, double* tri, int64_t capacity, int64_t* ntri) {
+    return guarded(nullptr, [&] {
+        if (!path || !ntri) fail(PVLBM_EINVAL, "path or ntri is NULL");
+        std::ifstream file(path, std::ios::binary);
+        if (!file.is_open()) fail(PVLBM_EIO, "cannot open file %s", path);
+        // GeometryReader.cpp:119-176: 80-byte header, uint32 facet count, per facet float normal
+        // (normalised in float: n / sqrtf(nx*nx + ny*ny + nz*nz)), three float vertices and a
+        // uint16 attribute count that is ignored
+        char header[80];
+        uint32_t n = 0;
+        if (!file.read(header, 80) || !file.read(reinterpret_cast<char*>(&n), 4))
+            fail(PVLBM_EIO, "%s: shorter than a binary STL header", path);
+        for (uint32_t t = 0; t < n; t++) {
+            float f[12];
+            uint16_t attr;
+            if (!file.read(reinterpret_cast<char*>(f), 48) || !file.read(reinterpret_cast<char*>(&attr), 2))
+                fail(PVLBM_EIO, "%s: truncated after %u of %u facets", path, t, n);
+            float nx = f[0], ny = f[1], nz = f[2];
+            const float norm = std::sqrt(nx * nx + ny * ny + nz * nz);
+            nx /= norm;
+            ny /= norm;
+            nz /= norm;
+            if (tri && (int64_t)t < capacity) {
+                double* T = tri + 12 * (int64_t)t;
+                for (int v = 0; v < 9; v++) T[v] = (double)f[3 + v];
+                T[9] = nx;
+                T[10] = ny;
+                T[11] = nz;
+            }
+        }
+        *ntri = (int64_t)n;
+    });
+}
+
 extern "C" int pvlbm_bbox(const double* tri, int64_t ntri, double bbox[6]) {
     return guarded(nullptr, [&] {
         if (!tri || ntri <= 0 || !bbox) fail(PVLBM_EINVAL, "need at least one triangle");

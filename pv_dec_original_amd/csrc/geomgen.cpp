@@ -420,6 +420,24 @@ extern "C" int pvgen_write_stl_ascii(const char* path, const double* tri, int64_
     return std::fclose(fp) == 0 ? 0 : 1;
 }
 
+extern "C" int pvgen_write_stl_binary(const char* path, const double* tri, int64_t ntri, const char* name) {
+    if (ntri < 0 || ntri > (int64_t)UINT32_MAX) return 1;
+    FILE* fp = std::fopen(path, "wb");
+    if (!fp) return 1;
+    char header[80] = {0};
+    std::snprintf(header, sizeof header, "%s", name ? name : "pvgen");
+    const uint32_t n = (uint32_t)ntri;
+    bool ok = std::fwrite(header, 1, 80, fp) == 80 && std::fwrite(&n, 4, 1, fp) == 1;
+    for (int64_t t = 0; ok && t < ntri; t++) {
+        const double* T = tri + 12 * t;
+        float f[12] = {(float)T[9], (float)T[10], (float)T[11]};
+        for (int v = 0; v < 9; v++) f[3 + v] = (float)T[v];
+        const uint16_t attr = 0;
+        ok = std::fwrite(f, 4, 12, fp) == 12 && std::fwrite(&attr, 2, 1, fp) == 1;
+    }
+    return (std::fclose(fp) == 0 && ok) ? 0 : 1;
+}
+
 extern "C" int pvgen_field(int shape, const double* params, int64_t npts, const double* xyz,
                            double* phi) {
     if (shape != 1) return 1;

@@ -8,13 +8,14 @@
 //           [--storage fp64|fp16|bf16] [--tau 1.5] [--dt 8e-6]
 //           [--faces pv|x] [--face role,closure,axis,plane,tol,lo0,lo1,hi0,hi1,rho]...
 //           [--t-end 2.7] [--period 0.9] [--steps N --inlet const:V|sin:MEAN,AMP,PERIOD]
-//           [--output-every 100] [--outdir .]
+//           [--output-every 100] [--outdir .] [--vtu]
 //
 // --faces pv uses the reference's PV.stl planes (ComplexGeometry.cpp:1378-1408); --faces x puts the
 // inlet on the grid's x-min plane and a pressure outlet (rho 1) on its x-max plane.  Without
 // --steps the run is the reference loop: the pulsatile waveform until --t-end.  Every
 // --output-every iterations FlowHemodynamics/<iter>_FlowHemodynamics.csv is written in the
-// reference's format (OutputFile.cpp:867-938).
+// reference's format (OutputFile.cpp:867-938); with --vtu also Fluid_Variables/<n>.vtu, n counting
+// the outputs from 0 (OutputFile.cpp:259-429).
 
 #include <sys/stat.h>
 
@@ -68,12 +69,86 @@ void write_csv(const std::string& outdir, int64_t iter, const std::vector<double
             << rows[i + 4] << "," << rows[i + 5] << "\n";
 }
 
+// Fluid_Variables/<idx>.vtu in the content of OutputFile::WriteVTK_Collective
+// (OutputFile.cpp:259-429): fluid nodes at integer lattice indices in z-y-x loop order, point data
+// Velocity, Density, VelocityMagnitude, a hexahedron per cube of eight fluid corners; XML with raw
+// appended binary blocks (UInt64 byte count, then little-endian data).
+void write_vtu(const std::string& outdir, int idx, const std::vector<uint8_t>& kind, int nx, int ny, int nz,
+               const std::vector<double>& rho, const std::vector<double>& u) {
+    const size_t n = (size_t)nx * ny * nz;
+    std::vector<int64_t> pid(n, -1);
+    std::vector<double> vel, den, mag, pts;
+    for (size_t b = 0; b < n; b++)
+        if (kind[b] != PVLBM_SOLID) {
+            pid[b] = (int64_t)den.size();
+            const double v0 = u[b], v1 = u[n + b], v2 = u[2 * n + b];
+            vel.insert(vel.end(), {v0, v1, v2});
+            den.push_back(rho[b]);
+            mag.push_back(std::sqrt(v0 * v0 + v1 * v1 + v2 * v2));
+            pts.insert(pts.end(), {(double)(b % nx), (double)((b / nx) % ny), (double)(b / ((size_t)nx * ny))});
+        }
+    std::vector<int64_t> conn;
+    const int cx[8] = {0, 1, 1, 0, 0, 1, 1, 0}, cy[8] = {0, 0, 1, 1, 0, 0, 1, 1}, cz[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+    for (int z = 0; z + 1 < nz; z++)
+        for (int y = 0; y + 1 < ny; y++)
+            for (int x = 0; x + 1 < nx; x++) {
+                int64_t c[8];
+                bool ok = true;
+                for (int i = 0; i < 8 && ok; i++) {
+                    c[i] = pid[((size_t)(z + cz[i]) * ny + (y + cy[i])) * nx + (x + cx[i])];
+                    ok = c[i] >= 0;
+                }
+                if (ok) conn.insert(conn.end(), c, c + 8);
+            }
+    const size_t m = conn.size() / 8;
+    std::vector<int64_t> offs(m);
+    for (size_t i = 0; i < m; i++) offs[i] = 8 * (int64_t)(i + 1);
+    std::vector<uint8_t> types(m, 12);  // VTK_HEXAHEDRON
+    struct Blk { const char* name; const char* type; int nc; const void* data; size_t bytes; };
+    const Blk blk[7] = {{"Velocity", "Float64", 3, vel.data(), vel.size() * 8},
+                        {"Density", "Float64", 1, den.data(), den.size() * 8},
+                        {"VelocityMagnitude", "Float64", 1, mag.data(), mag.size() * 8},
+                        {nullptr, "Float64", 3, pts.data(), pts.size() * 8},
+                        {"connectivity", "Int64", 1, conn.data(), conn.size() * 8},
+                        {"offsets", "Int64", 1, offs.data(), offs.size() * 8},
+                        {"types", "UInt8", 1, types.data(), types.size()}};
+    size_t off[7], o = 0;
+    for (int i = 0; i < 7; i++) {
+        off[i] = o;
+        o += 8 + blk[i].bytes;
+    }
+    auto da = [&](int i) {
+        std::string s = std::string("<DataArray type=\"") + blk[i].type + "\"";
+        if (blk[i].name) s += std::string(" Name=\"") + blk[i].name + "\"";
+        if (blk[i].nc > 1) s += " NumberOfComponents=\"" + std::to_string(blk[i].nc) + "\"";
+        return s + " format=\"appended\" offset=\"" + std::to_string(off[i]) + "\"/>";
+    };
+    std::string folder = outdir + "/Fluid_Variables";
+    mkdir(outdir.c_str(), 0777);
+    mkdir(folder.c_str(), 0777);
+    std::ofstream out(folder + "/" + std::to_string(idx) + ".vtu", std::ios::binary);
+    out << "<?xml version=\"1.0\"?>\n"
+        << "<VTKFile type=\"UnstructuredGrid\" version=\"1.0\" byte_order=\"LittleEndian\" header_type=\"UInt64\">\n"
+        << "  <UnstructuredGrid>\n"
+        << "    <Piece NumberOfPoints=\"" << den.size() << "\" NumberOfCells=\"" << m << "\">\n"
+        << "      <PointData>\n        " << da(0) << "\n        " << da(1) << "\n        " << da(2)
+        << "\n      </PointData>\n      <Points>\n        " << da(3) << "\n      </Points>\n      <Cells>\n        "
+        << da(4) << "\n        " << da(5) << "\n        " << da(6) << "\n      </Cells>\n    </Piece>\n"
+        << "  </UnstructuredGrid>\n  <AppendedData encoding=\"raw\">\n   _";
+    for (int i = 0; i < 7; i++) {
+        const uint64_t nb = blk[i].bytes;
+        out.write(reinterpret_cast<const char*>(&nb), 8);
+        out.write(static_cast<const char*>(blk[i].data), (std::streamsize)nb);
+    }
+    out << "\n  </AppendedData>\n</VTKFile>\n";
+}
+
 int usage() {
     std::cerr << "usage: lbm_run --stl FILE [--spacing H] [--grid ox,oy,oz,nx,ny,nz] [--lattice 19|27]\n"
                  "               [--storage fp64|fp16|bf16] [--tau T] [--dt DT] [--faces pv|x]\n"
                  "               [--face role,closure,axis,plane,tol,lo0,lo1,hi0,hi1,rho]...\n"
                  "               [--t-end T] [--period P] [--steps N --inlet const:V|sin:M,A,P]\n"
-                 "               [--output-every K] [--outdir DIR]\n";
+                 "               [--output-every K] [--outdir DIR] [--vtu]\n";
     return 2;
 }
 
@@ -83,6 +158,7 @@ int main(int argc, char** argv) {
     std::string stl, storage = "fp64", faces_kind, inlet = "const:0.05", outdir = ".";
     double spacing = 0.05, tau = 1.5, dt = 8e-6, t_end = 2.7, period = 0.9;
     int lattice = 19, output_every = 100;
+    bool vtu = false;
     long steps = -1;
     std::vector<double> grid;
     std::vector<pvlbm_face> faces;
@@ -111,6 +187,7 @@ int main(int argc, char** argv) {
             else if (k == "--inlet") inlet = val();
             else if (k == "--output-every") output_every = std::stoi(val());
             else if (k == "--outdir") outdir = val();
+            else if (k == "--vtu") vtu = true;
             else if (k == "--help" || k == "-h") return usage(), 0;
             else return usage();
         } catch (const std::exception& e) {
@@ -153,8 +230,16 @@ int main(int argc, char** argv) {
         std::cerr << "grid " << geo->grid.dims[0] << "x" << geo->grid.dims[1] << "x" << geo->grid.dims[2]
                   << ", nodes " << sd.count(PVLBM_LIST_FLUID) << ", inlet " << sd.count(PVLBM_LIST_INLET)
                   << ", wall " << sd.count(PVLBM_LIST_WALL) << ", next-to-wall " << sd.count(PVLBM_LIST_NTW) << "\n";
+        int vtu_idx = 0;
+        std::vector<uint8_t> kinds;
+        if (vtu) kinds = sd.kinds();
         auto out = [&](int64_t it) {
             write_csv(outdir, it, sim.surfaceAttributes());
+            if (vtu) {
+                std::vector<double> rho, u;
+                sim.macros(rho, u);
+                write_vtu(outdir, vtu_idx++, kinds, geo->grid.dims[0], geo->grid.dims[1], geo->grid.dims[2], rho, u);
+            }
             std::cerr << "iteration " << it << " written\n";
         };
         if (steps < 0) {

@@ -23,6 +23,40 @@ def read_stl_ascii(path):
     return tri
 
 
+def read_stl_binary(path):
+    """Triangles [T][12] of a binary STL, normals normalised in single precision
+    (GeometryReader.cpp:119-176)."""
+    n = C.c_int64(0)
+    N.check(N.lib().pvlbm_read_stl_binary(str(path).encode(), None, 0, C.byref(n)))
+    tri = np.zeros((n.value, 12), dtype=np.float64)
+    if n.value:
+        N.check(N.lib().pvlbm_read_stl_binary(str(path).encode(), N.dptr(tri), n.value, C.byref(n)))
+    return tri
+
+
+def is_binary_stl(path):
+    """A file is binary STL when its size is 84 + 50 * (the uint32 count at byte 80)."""
+    import os
+    size = os.path.getsize(path)
+    if size < 84:
+        return False
+    with open(path, "rb") as fp:
+        fp.seek(80)
+        n = int.from_bytes(fp.read(4), "little")
+    return size == 84 + 50 * n
+
+
+def read_stl(path):
+    """Binary or ASCII STL by content (the reference reads ASCII, ComplexGeometry.cpp:135)."""
+    return read_stl_binary(path) if is_binary_stl(path) else read_stl_ascii(path)
+
+
+def write_stl_binary(path, tri, name="pvlbm"):
+    tri = np.ascontiguousarray(tri, dtype=np.float64)
+    if N.gen().pvgen_write_stl_binary(str(path).encode(), N.dptr(tri), len(tri), name.encode()) != 0:
+        raise OSError(f"cannot write {path}")
+
+
 def write_stl_ascii(path, tri, name="pvlbm"):
     tri = np.ascontiguousarray(tri, dtype=np.float64)
     if N.gen().pvgen_write_stl_ascii(str(path).encode(), N.dptr(tri), len(tri), name.encode()) != 0:
@@ -75,7 +109,7 @@ class Geometry:
     """
 
     def __init__(self, FileName, velSet=None, spacing=0.05, raydirection='z', *, grid=None):
-        tri = read_stl_ascii(FileName)
+        tri = read_stl(FileName)
         if len(tri) == 0:
             raise ValueError(f"no facets read from {FileName}")
         self._setup(tri, velSet, spacing, raydirection, grid)

@@ -266,6 +266,17 @@ class Simulation:
         self._chk(N.lib().pvlbm_gate_events(self.handle, C.byref(n)))
         return n.value
 
+    def WriteVTK(self, index, outdir):
+        """Fluid_Variables/<index>.vtu: velocity, density and speed of every fluid node on the
+        lattice's hexahedra (OutputFile::WriteVTK_Collective, OutputFile.cpp:259-429)."""
+        from .output import write_fluid_vtu
+        folder = os.path.join(outdir, "Fluid_Variables")
+        os.makedirs(folder, exist_ok=True)
+        path = os.path.join(folder, f"{index}.vtu")
+        rho, u = self.macros()
+        write_fluid_vtu(path, self.SubDomain_.kinds() != N.SOLID, rho, u)
+        return path
+
     def WriteOutput(self, iteration, outdir):
         """FlowHemodynamics/<iter>_FlowHemodynamics.csv (OutputFile.cpp:867-938)."""
         from .output import write_surface_csv

@@ -33,6 +33,7 @@ def load():
     L = C.CDLL(LIB)
     sig = {
         "orc_read_stl_ascii": (C.c_int64, [C.c_char_p, _d, C.c_int64]),
+        "orc_read_stl_binary": (C.c_int64, [C.c_char_p, _d, C.c_int64]),
         "orc_bbox": (None, [_d, C.c_int64, _d]),
         "orc_grid_dims": (None, [_d, C.c_double, _i32]),
         "orc_point_in_triangle": (C.c_int, [C.c_double, C.c_double, C.c_double, _d]),
@@ -74,6 +75,15 @@ def read_stl(path):
     tri = np.zeros((max(n, 0), 12))
     if n > 0:
         L.orc_read_stl_ascii(str(path).encode(), _dp(tri), n)
+    return tri
+
+
+def read_stl_binary(path):
+    L = load()
+    n = L.orc_read_stl_binary(str(path).encode(), None, 0)
+    tri = np.zeros((max(n, 0), 12))
+    if n > 0:
+        L.orc_read_stl_binary(str(path).encode(), _dp(tri), n)
     return tri
 
 

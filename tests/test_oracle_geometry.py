@@ -160,3 +160,37 @@ def test_binned_labels_equal_brute_force(tube_tri):
         a = O.label_points(t, bb, dims, origin, 1.0)
         b = O.label_points(t, bb, dims, origin, 1.0, binned=True)
         assert np.array_equal(a, b)
+
+
+def test_binary_stl_round_trip_and_reference_normalisation(tmp_path):
+    """Binary STL (GeometryReader.cpp:119-176): vertices are the file's float32 values, normals
+    are normalised in float32 arithmetic (checked against numpy float32 ops), the library reader
+    matches the oracle bit for bit, and a truncated file is an error."""
+    import struct
+    from pv_dec_original_amd import geometry as G
+    from pv_dec_original_amd.workloads import vessel_mesh
+    tri = vessel_mesh(11)[:500].copy()
+    tri[0, 9:12] = (3.0, 4.0, 0.0)      # not unit length
+    tri[1, 9:12] = (0.0, 0.0, 0.0)      # zero normal -> NaN, as the reference computes it
+    path = tmp_path / "v.stl"
+    G.write_stl_binary(path, tri)
+    assert G.is_binary_stl(path) and path.stat().st_size == 84 + 50 * len(tri)
+    got = G.read_stl_binary(path)
+    ref = O.read_stl_binary(path)
+    assert got.shape == (500, 12) and np.array_equal(got, ref, equal_nan=True)
+    assert np.array_equal(got[:, :9], tri[:, :9].astype(np.float32).astype(np.float64))
+    n = tri[:, 9:12].astype(np.float32)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        norm = np.sqrt(n[:, 0] * n[:, 0] + n[:, 1] * n[:, 1] + n[:, 2] * n[:, 2])
+        want = (n / norm[:, None]).astype(np.float64)
+    assert np.array_equal(got[:, 9:12], want, equal_nan=True)
+    assert np.allclose(got[0, 9:12], [0.6, 0.8, 0.0], atol=1e-7) and np.isnan(got[1, 9:12]).all()
+    # Geometry reads binary files by content
+    assert np.array_equal(G.read_stl(path), got, equal_nan=True)
+    # truncated: the library refuses, the oracle keeps the complete facets
+    data = path.read_bytes()
+    (tmp_path / "t.stl").write_bytes(data[:84 + 50 * 10 + 20])
+    with pytest.raises(Exception):
+        G.read_stl_binary(tmp_path / "t.stl")
+    assert len(O.read_stl_binary(tmp_path / "t.stl")) == 10
+    assert struct.unpack("<I", data[80:84])[0] == 500
