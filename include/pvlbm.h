@@ -193,6 +193,29 @@ int pvlbm_init_fields_coarse(pvlbm_handle h, const float* rho_c, const float* u_
                              const int32_t cdims[3], double umax, double rho_lo, double rho_hi,
                              double flux_x, int64_t* nonfinite);
 
+/* Link fractions of the wall nodes (list PVLBM_LIST_WALL, list order): for each direction
+ * d < ndirs whose neighbour is in the box and solid, the segment from the node's point to point +
+ * c_d * h is tested against the triangles, and the first one in file order that it intersects
+ * gives q = distance / h.  q_out[nwall][Q]; -1 where there is no such link.  ndirs = 19 is the
+ * reference's loop (it hard-codes 19 directions, also for D3Q27); ndirs = Q covers every
+ * direction.  Replaces Geometry::CalculateWallDistances (ComplexGeometry.cpp:550-608) with
+ * FindLineTriangleIntersection (:689-730); the reference computes these and never uses them. */
+int pvlbm_link_fractions(pvlbm_handle h, int ndirs, double* q_out);
+
+/* Closure of the wall nodes.  PVLBM_WALL_REFERENCE (default): the reference's rule,
+ * f[opp d] = f[d] on the post-stream populations in direction order (Simulation.cpp:843-863).
+ * PVLBM_WALL_INTERPOLATED: linear interpolated bounce-back (Bouzidi, Firdaouss and Lallemand
+ * 2001) on each link from a wall node to a solid neighbour, with q the fraction of the link at
+ * which the segment from the node meets the surface first; the segment starts at the node
+ * shifted by h/2 in z, where the voxelisation tests the node (ComplexGeometry.cpp:989-1003).
+ * For q < 1/2: f_k = 2q f*_i(x) + (1 - 2q) f*_i(x - c_i); for q >= 1/2:
+ * f_k = f*_i(x)/(2q) + (2q - 1)/(2q) f*_k(x)  (c_i towards the wall, k = opp i, f* the
+ * post-collision populations); a link without a surface crossing, or whose other neighbour is
+ * solid too, takes halfway bounce-back f_k = f*_i(x).  No counterpart in the reference: it
+ * computes link fractions (ComplexGeometry.cpp:550-608) but never uses them. */
+enum { PVLBM_WALL_REFERENCE = 0, PVLBM_WALL_INTERPOLATED = 1 };
+int pvlbm_set_wall_closure(pvlbm_handle h, int mode);
+
 /* Every how many iterations WSS and its accumulators are evaluated (reference: every iteration,
  * Simulation.cpp:585-586).  Default 1. */
 int pvlbm_set_wss_every(pvlbm_handle h, int every);

@@ -988,6 +988,66 @@ extern "C" void orc_sim_normals(const orc_sim* s, double* n3) {
 
 extern "C" int64_t orc_sim_gate_events(const orc_sim* s) { return s->gate; }
 
+namespace {
+
+// Geometry::FindLineTriangleIntersection (ComplexGeometry.cpp:689-730).
+bool find_line_triangle(const double start[3], const double end[3], const double* TRI, double ip[3]) {
+    const double threshold = 1e-7;
+    double px = start[0], py = start[1], pz = start[2];
+    double dx = end[0] - start[0], dy = end[1] - start[1], dz = end[2] - start[2];
+    double nx = TRI[9], ny = TRI[10], nz = TRI[11];
+    double qx = TRI[0], qy = TRI[1], qz = TRI[2];
+    double denom = (dx * nx + dy * ny + dz * nz);
+    if (std::abs(denom) < threshold) return false;
+    double t = ((qx - px) * nx + (qy - py) * ny + (qz - pz) * nz) / denom;
+    if (t < 0.0 || t > 1.0) return false;
+    double ix = px + t * dx;
+    double iy = py + t * dy;
+    double iz = pz + t * dz;
+    if (point_in_triangle(ix, iy, iz, TRI)) {
+        ip[0] = ix;
+        ip[1] = iy;
+        ip[2] = iz;
+        return true;
+    }
+    return false;
+}
+
+}  // namespace
+
+// Geometry::CalculateWallDistances (ComplexGeometry.cpp:550-608) over the wall nodes in the
+// oracle's list order: for dir < ndirs (the reference hard-codes 19), a neighbour in the box that
+// is not fluid gives the segment point -> point + c * h, and the first triangle in file order that
+// it intersects gives q = distance / h.  box_out[w] = box index, q_out[w][q] (-1 without a link).
+extern "C" void orc_sim_link_fractions(const orc_sim* s, const double* tri, int64_t ntri, int ndirs,
+                                       int64_t* box_out, double* q_out) {
+    const std::vector<int32_t>& W = s->lists[PVLBM_LIST_WALL];
+    for (size_t w = 0; w < W.size(); w++) {
+        const int64_t b = s->node_box[W[w]];
+        const int64_t i = b % s->nx, j = (b / s->nx) % s->ny, k = b / (s->nx * s->ny);
+        box_out[w] = b;
+        for (int d = 0; d < s->q; d++) q_out[w * s->q + d] = -1.0;
+        for (int dir = 0; dir < ndirs; ++dir) {
+            const int64_t xn = i + s->L.c[dir][0], yn = j + s->L.c[dir][1], zn = k + s->L.c[dir][2];
+            // solid = a box point that is not a node
+            if (!in_box(s, xn, yn, zn) || s->box_node[(size_t)(xn + s->nx * (yn + s->ny * zn))] >= 0) continue;
+            double start[3] = {s->ox + (double)i * s->h, s->oy + (double)j * s->h, s->oz + (double)k * s->h};
+            double link[3] = {s->L.c[dir][0] * s->h, s->L.c[dir][1] * s->h, s->L.c[dir][2] * s->h};
+            double end[3] = {start[0] + link[0], start[1] + link[1], start[2] + link[2]};
+            for (int64_t t = 0; t < ntri; t++) {
+                double ip[3];
+                if (find_line_triangle(start, end, tri + 12 * t, ip)) {
+                    double distanceToIntersection =
+                        std::sqrt((ip[0] - start[0]) * (ip[0] - start[0]) + (ip[1] - start[1]) * (ip[1] - start[1]) +
+                                  (ip[2] - start[2]) * (ip[2] - start[2]));
+                    q_out[w * s->q + dir] = distanceToIntersection / s->h;
+                    break;
+                }
+            }
+        }
+    }
+}
+
 extern "C" void orc_collide(int q, double tau, const double* f, double* feq, double* fnew,
                             double* out4) {
     Lat L = lattice(q);

@@ -85,6 +85,11 @@ void orc_sim_wss(const orc_sim* s, double* out6, double* tau3, double* acc4);
 void orc_sim_normals(const orc_sim* s, double* n3);
 int64_t orc_sim_gate_events(const orc_sim* s);
 
+/* Geometry::CalculateWallDistances (ComplexGeometry.cpp:550-608) with FindLineTriangleIntersection
+ * (:689-730): per wall node (oracle list order) its box index and q per direction (-1: no link). */
+void orc_sim_link_fractions(const orc_sim* s, const double* tri, int64_t ntri, int ndirs,
+                            int64_t* box_out, double* q_out);
+
 /* One node's Simulation::Collision (Simulation.cpp:52-205): out4 = (rho, ux, uy, uz) of f,
  * feq = its equilibrium, fnew = the relaxed populations. */
 void orc_collide(int q, double tau, const double* f, double* feq, double* fnew, double* out4);

@@ -130,6 +130,12 @@ struct pvlbm_sim {
     double* d_partials = nullptr;
     double* d_red = nullptr;
     unsigned long long* d_gate = nullptr;
+    // host copy of the triangles (link fractions), boundary slots (padded node, closure)
+    std::vector<double> tri;
+    std::vector<int64_t> bnode;
+    std::vector<uint8_t> bclose;
+    int wall_mode = PVLBM_WALL_REFERENCE;
+    float* d_bq = nullptr;
     // host-side lists (unpadded box indices) and slots
     std::vector<int64_t> lists[8];
     std::vector<int32_t> inlet_slot;
@@ -150,7 +156,7 @@ struct pvlbm_sim {
 
     ~pvlbm_sim() {
         if (stream) (void)hipStreamSynchronize(stream);
-        void* ptrs[] = {d_kind, d_aux, d_tdesc, d_pop[0], d_pop[1], d_bmask, d_bclose, d_bprof, d_brho,
+        void* ptrs[] = {d_bq, d_kind, d_aux, d_tdesc, d_pop[0], d_pop[1], d_bmask, d_bclose, d_bprof, d_brho,
                         d_M, d_nrm, d_wss, d_tau3, d_acc, d_macro, d_partials, d_red, d_gate};
         for (void* p : ptrs) dfree(p);
         if (ev0) (void)hipEventDestroy(ev0);
@@ -174,6 +180,7 @@ struct pvlbm_sim {
         p.bclose = d_bclose;
         p.bprof = d_bprof;
         p.brho = d_brho;
+        p.bq = wall_mode == PVLBM_WALL_INTERPOLATED ? d_bq : nullptr;
         p.M = d_M;
         p.mst = std::max<int64_t>(nw, 1);
         p.nrm = d_nrm;
@@ -590,6 +597,8 @@ extern "C" int pvlbm_create(const uint8_t* fluid, const pvlbm_grid* grid, double
             }
         }
         S->nb = (int64_t)bnode.size();
+        S->bnode = bnode;
+        S->bclose = bclose;
 
         // next-to-wall marking on the GPU
         S->d_kind = dalloc<uint8_t>((size_t)nsp);
@@ -706,6 +715,7 @@ extern "C" int pvlbm_create(const uint8_t* fluid, const pvlbm_grid* grid, double
                 d_xyz = dalloc<double>((size_t)(3 * S->nw));
                 HIPCHK(hipMemcpyAsync(d_tri, tri, sizeof(double) * 12 * ntri, hipMemcpyHostToDevice, st));
                 HIPCHK(hipMemcpyAsync(d_xyz, xyz.data(), sizeof(double) * 3 * S->nw, hipMemcpyHostToDevice, st));
+                S->tri.assign(tri, tri + 12 * ntri);
                 HIPCHK(launch_tri_prep(d_tri, ntri, d_prep, st));
                 HIPCHK(launch_normals(d_prep, ntri, d_xyz, S->nw, S->d_nrm, st));
                 HIPCHK(hipStreamSynchronize(st));
@@ -854,6 +864,165 @@ extern "C" int pvlbm_init_fields_coarse(pvlbm_handle h, const float* rho_c, cons
         }
         dfree(d_in);
         dfree(d_cnt);
+    });
+}
+
+namespace {
+
+// Link fractions of (node, direction) pairs: nodes are unpadded box indices; for d < ndirs whose
+// neighbour is in the box and solid, the segment from the node's point (z shifted by zshift) to
+// point + c_d h; q_out[node][Q] = distance / h of the first triangle in file order (nearest = 0,
+// the reference) or of the nearest intersection (nearest = 1), -1 without one.
+void link_q(pvlbm_sim* h, const std::vector<int64_t>& nodes, int ndirs, double zshift, int nearest,
+            double* q_out) {
+    const int64_t ntri = (int64_t)h->tri.size() / 12;
+    if (ntri == 0) fail(PVLBM_ESTATE, "the handle was created without triangles");
+    const int Q = h->q;
+    for (size_t i = 0; i < nodes.size() * (size_t)Q; i++) q_out[i] = -1.0;
+    std::vector<uint8_t> k((size_t)h->nsp);
+    HIPCHK(hipMemcpyAsync(k.data(), h->d_kind, (size_t)h->nsp, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const int* cx = Q == 19 ? Lat<19>::cx : Lat<27>::cx;
+    const int* cy = Q == 19 ? Lat<19>::cy : Lat<27>::cy;
+    const int* cz = Q == 19 ? Lat<19>::cz : Lat<27>::cz;
+    // (node, direction) pairs whose neighbour is in the box and solid (ComplexGeometry.cpp:559-572)
+    std::vector<double> seg;
+    std::vector<int64_t> where;
+    for (size_t w = 0; w < nodes.size(); w++) {
+        const int64_t b = nodes[w], i = b % h->nx, j = (b / h->nx) % h->ny, kk = b / (h->nx * h->ny);
+        for (int d = 0; d < ndirs; d++) {
+            const int64_t a = i + cx[d], bb = j + cy[d], c = kk + cz[d];
+            if (a < 0 || a >= h->nx || bb < 0 || bb >= h->ny || c < 0 || c >= h->nz) continue;
+            if (k[(size_t)(a + h->nxp * (bb + h->nyp * c))] != 0) continue;
+            // line.start = the point (min + idx * h), line.end = start + c * h
+            const double sx = h->origin[0] + (double)i * h->h, sy = h->origin[1] + (double)j * h->h,
+                         sz = h->origin[2] + (double)kk * h->h + zshift;
+            const double lx = cx[d] * h->h, ly = cy[d] * h->h, lz = cz[d] * h->h;
+            seg.insert(seg.end(), {sx, sy, sz, sx + lx, sy + ly, sz + lz});
+            where.push_back((int64_t)w * Q + d);
+        }
+    }
+    const int64_t np = (int64_t)where.size();
+    if (np == 0) return;
+    // triangles binned into cells of 2h over the box with a margin (bounding boxes widened
+    // by 1e-6 of the geometry's scale against rounding at cell borders)
+    const double cs = 2.0 * h->h;
+    const double og[3] = {h->origin[0] - cs, h->origin[1] - cs, h->origin[2] - cs};
+    const int64_t dims[3] = {h->nx, h->ny, h->nz};
+    int32_t nc[3];
+    for (int a = 0; a < 3; a++) nc[a] = (int32_t)((double)(dims[a] + 3) * h->h / cs) + 2;
+    double scale = 1.0;
+    for (int64_t t = 0; t < ntri * 12; t++) scale = std::max(scale, std::fabs(h->tri[(size_t)t]));
+    const double m = 1e-6 * scale;
+    const int64_t ncell = (int64_t)nc[0] * nc[1] * nc[2];
+    auto range = [&](int64_t t, int r0[3], int r1[3]) {
+        const double* T = &h->tri[(size_t)(12 * t)];
+        for (int a = 0; a < 3; a++) {
+            double lo = std::min(T[a], std::min(T[3 + a], T[6 + a])) - m;
+            double hi = std::max(T[a], std::max(T[3 + a], T[6 + a])) + m;
+            r0[a] = (int)std::min<double>(std::max(std::floor((lo - og[a]) / cs), 0.0), nc[a] - 1);
+            r1[a] = (int)std::min<double>(std::max(std::floor((hi - og[a]) / cs), 0.0), nc[a] - 1);
+        }
+    };
+    // counting sort of (cell, triangle) entries; triangles ascending within a cell
+    std::vector<int64_t> count((size_t)ncell + 1, 0);
+    for (int64_t t = 0; t < ntri; t++) {
+        int r0[3], r1[3];
+        range(t, r0, r1);
+        for (int z = r0[2]; z <= r1[2]; z++)
+            for (int y = r0[1]; y <= r1[1]; y++)
+                for (int x = r0[0]; x <= r1[0]; x++) count[(size_t)(((int64_t)z * nc[1] + y) * nc[0] + x) + 1]++;
+    }
+    for (int64_t c = 0; c < ncell; c++) count[(size_t)c + 1] += count[(size_t)c];
+    if (count[(size_t)ncell] >= INT32_MAX) fail(PVLBM_EINVAL, "too many triangle-cell entries");
+    std::vector<int32_t> start(count.begin(), count.end());
+    std::vector<int32_t> cell_tri((size_t)std::max<int64_t>(count[(size_t)ncell], 1));
+    std::vector<int64_t> fill(count.begin(), count.end() - 1);
+    for (int64_t t = 0; t < ntri; t++) {
+        int r0[3], r1[3];
+        range(t, r0, r1);
+        for (int z = r0[2]; z <= r1[2]; z++)
+            for (int y = r0[1]; y <= r1[1]; y++)
+                for (int x = r0[0]; x <= r1[0]; x++)
+                    cell_tri[(size_t)fill[(size_t)(((int64_t)z * nc[1] + y) * nc[0] + x)]++] = (int32_t)t;
+    }
+    double *d_seg = nullptr, *d_tri = nullptr, *d_q = nullptr;
+    int32_t *d_start = nullptr, *d_ct = nullptr;
+    try {
+        d_seg = dalloc<double>(seg.size());
+        d_tri = dalloc<double>(h->tri.size());
+        d_q = dalloc<double>((size_t)np);
+        d_start = dalloc<int32_t>(start.size());
+        d_ct = dalloc<int32_t>(cell_tri.size());
+        HIPCHK(hipMemcpyAsync(d_seg, seg.data(), sizeof(double) * seg.size(), hipMemcpyHostToDevice, h->stream));
+        HIPCHK(hipMemcpyAsync(d_tri, h->tri.data(), sizeof(double) * h->tri.size(), hipMemcpyHostToDevice, h->stream));
+        HIPCHK(hipMemcpyAsync(d_start, start.data(), sizeof(int32_t) * start.size(), hipMemcpyHostToDevice, h->stream));
+        HIPCHK(hipMemcpyAsync(d_ct, cell_tri.data(), sizeof(int32_t) * cell_tri.size(), hipMemcpyHostToDevice, h->stream));
+        HIPCHK(launch_link_q(d_seg, np, d_tri, d_start, d_ct, og, cs, nc, h->h, nearest, d_q, h->stream));
+        std::vector<double> q((size_t)np);
+        HIPCHK(hipMemcpyAsync(q.data(), d_q, sizeof(double) * (size_t)np, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        for (int64_t p = 0; p < np; p++) q_out[where[(size_t)p]] = q[(size_t)p];
+    } catch (...) {
+        (void)hipStreamSynchronize(h->stream);
+        dfree(d_seg);
+        dfree(d_tri);
+        dfree(d_q);
+        dfree(d_start);
+        dfree(d_ct);
+        throw;
+    }
+    dfree(d_seg);
+    dfree(d_tri);
+    dfree(d_q);
+    dfree(d_start);
+    dfree(d_ct);
+}
+
+}  // namespace
+
+extern "C" int pvlbm_link_fractions(pvlbm_handle h, int ndirs, double* q_out) {
+    if (!h) return PVLBM_EINVAL;
+    return guarded(h, [&] {
+        if (!q_out) fail(PVLBM_EINVAL, "q_out is NULL");
+        if (ndirs < 1 || ndirs > h->q) fail(PVLBM_EINVAL, "ndirs must be in [1, Q]");
+        link_q(h, h->lists[PVLBM_LIST_WALL], ndirs, 0.0, 0, q_out);
+    });
+}
+
+extern "C" int pvlbm_set_wall_closure(pvlbm_handle h, int mode) {
+    if (!h) return PVLBM_EINVAL;
+    return guarded(h, [&] {
+        if (mode != PVLBM_WALL_REFERENCE && mode != PVLBM_WALL_INTERPOLATED) fail(PVLBM_EINVAL, "unknown wall closure");
+        if (mode == PVLBM_WALL_INTERPOLATED && !h->d_bq) {
+            // fractions of every link of every wall slot, per link length, nearest crossing of
+            // the segment shifted like the voxelisation's test points
+            const int Q = h->q;
+            std::vector<int64_t> nodes;
+            std::vector<int32_t> slot;
+            for (size_t sl = 0; sl < h->bnode.size(); sl++)
+                if (h->bclose[sl] == BC_WALL) {
+                    nodes.push_back(h->box_of_pad(h->bnode[sl]));
+                    slot.push_back((int32_t)sl);
+                }
+            std::vector<double> q(nodes.size() * (size_t)Q);
+            if (!nodes.empty()) link_q(h, nodes, Q, 0.5 * h->h, 1, q.data());
+            const int* cx = Q == 19 ? Lat<19>::cx : Lat<27>::cx;
+            const int* cy = Q == 19 ? Lat<19>::cy : Lat<27>::cy;
+            const int* cz = Q == 19 ? Lat<19>::cz : Lat<27>::cz;
+            std::vector<float> bq(std::max<size_t>(h->bnode.size(), 1) * (size_t)Q, -1.0f);
+            for (size_t w = 0; w < nodes.size(); w++)
+                for (int d = 0; d < Q; d++) {
+                    const double v = q[w * Q + d];
+                    if (v < 0.0) continue;
+                    const double len = std::sqrt((double)(cx[d] * cx[d] + cy[d] * cy[d] + cz[d] * cz[d]));
+                    bq[(size_t)slot[w] * Q + d] = (float)std::min(1.0, v / len);
+                }
+            h->d_bq = dalloc<float>(bq.size());
+            HIPCHK(hipMemcpyAsync(h->d_bq, bq.data(), sizeof(float) * bq.size(), hipMemcpyHostToDevice, h->stream));
+            HIPCHK(hipStreamSynchronize(h->stream));
+        }
+        h->wall_mode = mode;
     });
 }
 

@@ -66,6 +66,31 @@ __device__ __forceinline__ bool crosses(const double* T, const double s[3], cons
     return false;
 }
 
+// Geometry::FindLineTriangleIntersection (ComplexGeometry.cpp:689-730), same operation order:
+// plane of the file normal through vertex 1, |denominator| >= 1e-7, 0 <= t <= 1, then the
+// point-in-triangle test of the voxeliser.
+__device__ __forceinline__ bool line_triangle(const double* T, const double* s, const double* e, double ip[3]) {
+    const double threshold = 1e-7;
+    double px = s[0], py = s[1], pz = s[2];
+    double dx = e[0] - s[0], dy = e[1] - s[1], dz = e[2] - s[2];
+    double nx = T[9], ny = T[10], nz = T[11];
+    double qx = T[0], qy = T[1], qz = T[2];
+    double denom = (dx * nx + dy * ny + dz * nz);
+    if (fabs(denom) < threshold) return false;
+    double t = ((qx - px) * nx + (qy - py) * ny + (qz - pz) * nz) / denom;
+    if (t < 0.0 || t > 1.0) return false;
+    double ix = px + t * dx;
+    double iy = py + t * dy;
+    double iz = pz + t * dz;
+    if (point_in_triangle(ix, iy, iz, T)) {
+        ip[0] = ix;
+        ip[1] = iy;
+        ip[2] = iz;
+        return true;
+    }
+    return false;
+}
+
 // One wave per (i, j) column; its lanes walk the column's points in z.  The candidate triangles
 // of a column are the same for every lane, so their loads are wave-uniform.
 __global__ __launch_bounds__(64) void k_voxelize(const VoxParams p) {
@@ -255,6 +280,66 @@ __global__ __launch_bounds__(256) void k_copy(const double2* __restrict__ src, d
 }
 
 }  // namespace
+
+// link fractions: one thread per segment, candidates from the cells its bounding box touches
+__global__ void k_link_q(const double* seg, int64_t npairs, const double* tri, const int32_t* cell_start,
+                         const int32_t* cell_tri, double ogx, double ogy, double ogz, double cs, int ncx, int ncy,
+                         int ncz, double h, int nearest, double* q) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npairs) return;
+    const double* s = seg + 6 * p;
+    const double* e = s + 3;
+    const double og[3] = {ogx, ogy, ogz};
+    const int nc[3] = {ncx, ncy, ncz};
+    int c0[3], c1[3];
+    for (int a = 0; a < 3; a++) {
+        const double lo = fmin(s[a], e[a]) - 1e-9 * cs, hi = fmax(s[a], e[a]) + 1e-9 * cs;
+        c0[a] = min(max((int)floor((lo - og[a]) / cs), 0), nc[a] - 1);
+        c1[a] = min(max((int)floor((hi - og[a]) / cs), 0), nc[a] - 1);
+    }
+    // nearest == 0: the triangle of smallest index (the reference's first in file order);
+    // nearest == 1: the intersection closest to the start (the wall the link meets first)
+    int32_t best = INT32_MAX;
+    double bestd = 1e300;
+    double ip[3] = {0.0, 0.0, 0.0};
+    for (int cz = c0[2]; cz <= c1[2]; cz++)
+        for (int cy = c0[1]; cy <= c1[1]; cy++)
+            for (int cx = c0[0]; cx <= c1[0]; cx++) {
+                const int64_t c = ((int64_t)cz * nc[1] + cy) * nc[0] + cx;
+                for (int32_t k = cell_start[c]; k < cell_start[c + 1]; k++) {
+                    const int32_t t = cell_tri[k];
+                    if (!nearest && t >= best) break;  // ascending within a cell
+                    double hit[3];
+                    if (line_triangle(tri + 12 * (int64_t)t, s, e, hit)) {
+                        const double dd = (hit[0] - s[0]) * (hit[0] - s[0]) + (hit[1] - s[1]) * (hit[1] - s[1]) +
+                                          (hit[2] - s[2]) * (hit[2] - s[2]);
+                        if (nearest && (dd > bestd || (dd == bestd && t > best))) continue;
+                        best = t;
+                        bestd = dd;
+                        ip[0] = hit[0];
+                        ip[1] = hit[1];
+                        ip[2] = hit[2];
+                        if (!nearest) break;
+                    }
+                }
+            }
+    if (best == INT32_MAX) {
+        q[p] = -1.0;
+        return;
+    }
+    const double d = sqrt((ip[0] - s[0]) * (ip[0] - s[0]) + (ip[1] - s[1]) * (ip[1] - s[1]) +
+                          (ip[2] - s[2]) * (ip[2] - s[2]));
+    q[p] = d / h;
+}
+
+hipError_t launch_link_q(const double* seg, int64_t npairs, const double* tri, const int32_t* cell_start,
+                         const int32_t* cell_tri, const double og[3], double cs, const int32_t nc[3], double h,
+                         int nearest, double* q, hipStream_t s) {
+    if (npairs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_link_q, dim3((unsigned)((npairs + 255) / 256)), dim3(256), 0, s, seg, npairs, tri,
+                       cell_start, cell_tri, og[0], og[1], og[2], cs, nc[0], nc[1], nc[2], h, nearest, q);
+    return hipGetLastError();
+}
 
 // coarse-field expansion for starts from a network prediction
 __device__ __forceinline__ float coarse_coord(int64_t i, int64_t n, int c, int& i0, int& i1) {

@@ -108,6 +108,35 @@ __device__ __forceinline__ void save_stale(const StepParams& p, uint32_t nb, uin
     }
 }
 
+// Interpolated bounce-back on the wall links of a wall node (Bouzidi, Firdaouss and Lallemand
+// 2001, linear): for each population D with no upstream node, I = opp D points at the wall and
+// q = bq[slot][I] is where the link meets it.  f*(x) is this node's post-collision population of
+// the previous launch, read from its own slot of A; f*_I(x - c_I) is the population it pulled in
+// direction I (valid when x - c_I, i.e. x + c_D, is a node).
+template <int Q, class ST, int D = 0>
+__device__ __forceinline__ void wall_interpolated(const StepParams& p, uint32_t nb, uint32_t nmiss,
+                                                  const float* bq, typename ST::R (&f)[Q]) {
+    if constexpr (D < Q) {
+        using R = typename ST::R;
+        using S = typename ST::S;
+        constexpr int I = Lat<Q>::opp[D];
+        if ((nmiss >> I) & 1u) {
+            const char* base = static_cast<const char*>(p.A) + nb;
+            const R aI = ST::load(*reinterpret_cast<const S*>(base + I * p.ns * (int64_t)sizeof(S)), Lat<Q>::w[I]);
+            const R q = R(bq[I]);
+            R v = aI;  // no crossing found, or a thin wall: halfway bounce-back
+            if (q >= R(0.5)) {
+                const R aD = ST::load(*reinterpret_cast<const S*>(base + D * p.ns * (int64_t)sizeof(S)), Lat<Q>::w[D]);
+                v = aI / (R(2) * q) + (R(2) * q - R(1)) / (R(2) * q) * aD;
+            } else if (q >= R(0) && !((nmiss >> D) & 1u)) {
+                v = R(2) * q * aI + (R(1) - R(2) * q) * f[I];
+            }
+            f[D] = v;
+        }
+        wall_interpolated<Q, ST, D + 1>(p, nb, nmiss, bq, f);
+    }
+}
+
 template <int Q, class ST, int D = 0>
 __device__ __forceinline__ void collide_store(const StepParams& p, uint32_t nb, typename ST::R rho,
                                               const typename ST::R (&u)[3],
@@ -173,7 +202,10 @@ __device__ __forceinline__ bool node_update(const StepParams& p, double (&red)[4
             // closures in the reference's call order (Simulation.cpp:582-584); one per node
             cl = p.bclose[bslot];
             if (cl == BC_WALL) {
-                wall_rule<Q>(f, nmiss);
+                if (p.bq)
+                    wall_interpolated<Q, ST>(p, nb, nmiss, p.bq + (int64_t)bslot * Q, f);
+                else
+                    wall_rule<Q>(f, nmiss);
             } else if (cl == BC_INLET_XMIN) {
                 inlet_xmin(f, R(p.scale * p.bprof[bslot]));
             } else if (cl == BC_OUT_XMAX) {

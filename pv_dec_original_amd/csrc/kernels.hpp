@@ -47,6 +47,7 @@ struct StepParams {
     const uint8_t* bclose;  // enum BClose
     const double* bprof;    // inlet velocity profile
     const double* brho;     // prescribed outlet density
+    const float* bq;        // [slot][Q] wall-link fractions for the interpolated closure, or NULL
     // next-to-wall table
     void* M;                // previous (rho, ux, uy, uz) [4][mst] (arithmetic type)
     int64_t mst;
@@ -93,6 +94,17 @@ hipError_t launch_tri_prep(const double* tri, int64_t ntri, double* prep /*[8][n
 // area/d^2-weighted normal of each listed node (Subdomain.cpp:301-369)
 hipError_t launch_normals(const double* prep, int64_t ntri, const double* xyz /*[3][n]*/, int64_t n,
                           double* nrm /*[3][n]*/, hipStream_t s);
+
+// Link fractions of wall links (Geometry::CalculateWallDistances, ComplexGeometry.cpp:550-608):
+// for each segment seg[6*p .. 6*p+5] = (start, end), the triangle of smallest file index that
+// FindLineTriangleIntersection (:689-730) accepts (the reference takes the first in file order)
+// gives q = |intersection - start| / h; q = -1 without one (nearest = 1: the intersection
+// closest to the start instead).  Candidates come from a uniform grid of cells (size cs, origin
+// og, nc cells per axis) in CSR form: cell c holds cell_tri[cell_start[c] .. cell_start[c+1]),
+// ascending.
+hipError_t launch_link_q(const double* seg, int64_t npairs, const double* tri, const int32_t* cell_start,
+                         const int32_t* cell_tri, const double og[3], double cs, const int32_t nc[3], double h,
+                         int nearest, double* q, hipStream_t s);
 
 // streaming copy of `bytes` (multiple of 16) for counter calibration
 hipError_t launch_copy(const void* src, void* dst, int64_t bytes, hipStream_t s);

@@ -107,6 +107,15 @@ class SubDomain:
     def nextToTheWall(self):
         return self.list(N.LIST_NTW)
 
+    def link_fractions(self, ndirs=19):
+        """q [nwall][Q] of the wall nodes (list order), -1 where a direction has no wall link;
+        ndirs = 19 is the reference's loop, ndirs = Q every direction
+        (ComplexGeometry.cpp:550-608)."""
+        q = np.zeros((self.count(N.LIST_WALL), self.GetVelSet().Get_nDirections()))
+        if len(q):
+            self._chk(N.lib().pvlbm_link_fractions(self.handle, int(ndirs), N.dptr(q)))
+        return q
+
     def kinds(self):
         k = np.zeros((self.nz, self.ny, self.nx), dtype=np.uint8)
         self._chk(N.lib().pvlbm_get_kinds(self.handle, N.u8ptr(k)))
@@ -148,6 +157,11 @@ class Simulation:
         if len(u) != self.SubDomain_.count(N.LIST_INLET):
             raise ValueError("one value per inlet node is required")
         self._chk(N.lib().pvlbm_set_inlet_profile(self.handle, N.dptr(u)))
+
+    def set_wall_closure(self, mode):
+        """N.WALL_REFERENCE (the reference's wall rule, default) or N.WALL_INTERPOLATED (linear
+        interpolated bounce-back on the wall links' surface fractions)."""
+        self._chk(N.lib().pvlbm_set_wall_closure(self.handle, int(mode)))
 
     def set_wss_every(self, every):
         self._chk(N.lib().pvlbm_set_wss_every(self.handle, int(every)))

@@ -34,6 +34,7 @@ def load():
     sig = {
         "orc_read_stl_ascii": (C.c_int64, [C.c_char_p, _d, C.c_int64]),
         "orc_read_stl_binary": (C.c_int64, [C.c_char_p, _d, C.c_int64]),
+        "orc_sim_link_fractions": (None, [C.c_void_p, _d, C.c_int64, C.c_int, _i64, _d]),
         "orc_bbox": (None, [_d, C.c_int64, _d]),
         "orc_grid_dims": (None, [_d, C.c_double, _i32]),
         "orc_point_in_triangle": (C.c_int, [C.c_double, C.c_double, C.c_double, _d]),
@@ -221,6 +222,15 @@ class OracleSim:
         if nw:
             self.L.orc_sim_normals(self.h, _dp(n3))
         return n3
+
+    def link_fractions(self, tri, ndirs=19):
+        """(box index per wall node, q [nwall][Q]) in the oracle's wall-list order."""
+        nw = self.counts()["wall"]
+        box = np.zeros(nw, dtype=np.int64)
+        q = np.zeros((nw, self.q))
+        tri = np.ascontiguousarray(tri, dtype=np.float64)
+        self.L.orc_sim_link_fractions(self.h, _dp(tri), len(tri), ndirs, box.ctypes.data_as(_i64), _dp(q))
+        return box, q
 
     def gate_events(self):
         return self.L.orc_sim_gate_events(self.h)

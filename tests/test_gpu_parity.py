@@ -9,6 +9,7 @@ import pytest
 
 from pv_dec_original_amd import _native as N
 from pv_dec_original_amd.geometry import voxelize
+from pv_dec_original_amd.simulation import Simulation, SubDomain
 from tests import gpu_cases as G
 from tests import oracle_ctypes as O
 
@@ -181,3 +182,99 @@ def test_shifted_storage_error_vs_fp64(storage, tol):
     assert e_r < 10 * tol
     w64, w16 = sim64.wss()[:, 3], sim16.wss()[:, 3]
     assert np.linalg.norm(w16 - w64) / np.linalg.norm(w64) < 5 * tol
+
+
+@pytest.mark.parametrize("which", ["tube19", "ybif27", "ybif27all"])
+def test_link_fractions_match_oracle(which):
+    """Link fractions of the wall links (ComplexGeometry.cpp:550-608) bit for bit against the
+    oracle's brute-force first-in-file-order search; the device searches binned candidates."""
+    from pv_dec_original_amd.workloads import tube_case
+    case = tube_case(q=19) if which == "tube19" else G.small_ybif_case(q=27)
+    ndirs = 27 if which == "ybif27all" else 19
+    sd, sim, orc = G.build(case, normals=False)
+    q = sd.link_fractions(ndirs)
+    wall = sd.list(N.LIST_WALL)
+    box, qo = orc.link_fractions(case.geometry.globalTriangles, ndirs)
+    order = np.argsort(box)
+    assert np.array_equal(np.sort(wall), box[order])
+    qg = q[np.argsort(wall)]
+    qo = qo[order]
+    assert np.array_equal(qg, qo), (np.argwhere(qg != qo)[:5], qg[qg != qo][:5], qo[qg != qo][:5])
+    found = qg[qg >= 0]
+    # q is distance / h (not per link length), so diagonal links reach sqrt(2) or sqrt(3)
+    assert len(found) > 0 and found.max() <= np.sqrt(3.0) + 1e-12
+    if ndirs == 19 and case.geometry.m_velSet.q == 27:
+        assert (qg[:, 19:] == -1).all()
+
+
+def _tube_closure_errors(mode, steps=8000):
+    """Steady tube flow (R = 10, D3Q19, tau 0.8, axis at z = -h/2, where the voxelisation puts it)
+    with the given wall closure.  Returns the relative L2 distance of u_x on the mid plane from the
+    best-fitting Poiseuille parabola of radius 10, the relative L2 error of the physical WSS at the
+    next-to-wall nodes of the middle half against the exact Poiseuille stress at their radius,
+    rho nu 4 Q r / (pi R^4) with the flux Q through the mid plane, and the handle."""
+    from pv_dec_original_amd.workloads import tube_case
+    tau, R = 0.8, 10.0
+    case = tube_case(q=19, tau=tau)
+    geo = case.geometry
+    sd = SubDomain(geo, case.faces, storage="fp64", tau=case.tau, dt=G.DT)
+    sim = Simulation(sd, case.tau, 1.0, G.DT)
+    sim.set_inlet_profile(case.inlet_profile(sd, 0.05))
+    sim.set_wall_closure(mode)
+    sim.step(steps)
+    rho, u = sim.macros()
+    fl = sd.kinds() != N.SOLID
+    ix = geo.nx // 2
+    z, y = np.nonzero(fl[:, :, ix])
+    yc = geo.origin[1] + y * geo.spacing
+    zc = geo.origin[2] + z * geo.spacing
+    g = 1.0 - (yc ** 2 + (zc + 0.5) ** 2) / R ** 2
+    ux = u[0][z, y, ix]
+    c = (ux * g).sum() / (g * g).sum()
+    e_prof = np.linalg.norm(ux - c * g) / np.linalg.norm(ux)
+    Q = (rho[z, y, ix] * ux).sum() / rho[z, y, ix].mean()
+    w = sim.wss()
+    x0, x1 = geo.origin[0] + 0.25 * geo.nx, geo.origin[0] + 0.75 * geo.nx
+    mid = (w[:, 0] > x0) & (w[:, 0] < x1)
+    r = np.sqrt(w[mid, 1] ** 2 + (w[mid, 2] + 0.5) ** 2)
+    nu = (tau - 0.5) / 3.0
+    C = 1060 * G.DT ** 2 / (geo.spacing ** 2 * 1e-6)
+    phys = w[mid, 3] * (1 - 1 / (2 * tau)) / C  # the reference WSS rescaled to rho nu du/dr
+    exact = nu * 4 * Q * r / (np.pi * R ** 4)
+    e_wss = np.linalg.norm(phys - exact) / np.linalg.norm(exact)
+    return e_prof, e_wss, sim
+
+
+def test_interpolated_wall_closure_poiseuille():
+    """The interpolated closure puts no-slip on the surface: the steady tube profile is much
+    closer to the Poiseuille parabola of the STL radius than with the reference's wall rule, and
+    so is the wall shear stress at the next-to-wall nodes."""
+    e_ref, w_ref, sim_ref = _tube_closure_errors(N.WALL_REFERENCE)
+    e_int, w_int, sim = _tube_closure_errors(N.WALL_INTERPOLATED)
+    leak = []
+    for s in (sim_ref, sim):
+        red = s.reduce()
+        assert red[3] == 0
+        leak.append(abs(red[1] - red[2]) / abs(red[1]))  # inlet vs outlet mass flux
+    print(f"tube vs Poiseuille, reference wall rule / interpolated: profile {e_ref:.4f} / {e_int:.4f}, "
+          f"WSS {w_ref:.4f} / {w_int:.4f}, inlet-outlet flux mismatch {leak[0]:.4f} / {leak[1]:.4f}")
+    assert e_int < 0.5 * e_ref and e_int < 0.03
+    assert w_int < w_ref
+    # the inlet and outlet sums differ by a few percent for both closures (their rim nodes carry
+    # different wall links); the interpolated closure must not add to that
+    assert leak[1] <= leak[0] + 0.005
+
+
+def test_wall_closure_switch_back_is_the_reference_path():
+    """After computing the interpolation table and switching back, the step is bit-identical to a
+    handle that never switched."""
+    from pv_dec_original_amd.workloads import tube_case
+    out = []
+    for toggle in (False, True):
+        sd, sim, orc = G.build(tube_case(q=19), normals=False)
+        if toggle:
+            sim.set_wall_closure(N.WALL_INTERPOLATED)
+            sim.set_wall_closure(N.WALL_REFERENCE)
+        sim.step(150)
+        out.append(sim.macros())
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])

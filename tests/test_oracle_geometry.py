@@ -194,3 +194,64 @@ def test_binary_stl_round_trip_and_reference_normalisation(tmp_path):
         G.read_stl_binary(tmp_path / "t.stl")
     assert len(O.read_stl_binary(tmp_path / "t.stl")) == 10
     assert struct.unpack("<I", data[80:84])[0] == 500
+
+
+def _tilted_box(lo, hi, theta):
+    """Closed box [lo, hi] rotated by theta about the x axis: 12 triangles with outward normals,
+    and the 6 half-spaces n . p <= d."""
+    c, s = np.cos(theta), np.sin(theta)
+    R = np.array([[1, 0, 0], [0, c, -s], [0, s, c]])
+    corner = lambda b: R @ np.array([hi[0] if b & 1 else lo[0], hi[1] if b & 2 else lo[1], hi[2] if b & 4 else lo[2]])
+    quads = [(0, 2, 6, 4, -1, 0), (1, 5, 7, 3, 1, 0), (0, 4, 5, 1, -1, 1), (2, 3, 7, 6, 1, 1),
+             (0, 1, 3, 2, -1, 2), (4, 6, 7, 5, 1, 2)]
+    tris, planes = [], []
+    for a, b, cc, d, sign, ax in quads:
+        n = R @ (sign * np.eye(3)[ax])
+        P = [corner(v) for v in (a, b, cc, d)]
+        tris += [tri(P[0], P[1], P[2], n), tri(P[0], P[2], P[3], n)]
+        planes.append((n, n @ P[0]))
+    return np.array(tris), planes
+
+
+def test_link_fractions_closed_form_tilted_box():
+    """Wall-link fractions (ComplexGeometry.cpp:550-608) on a convex box tilted 0.3 rad about x:
+    a link from an inside node crosses the surface once, at t* = min over the faces it leaves
+    through of (d - n.p) / (n.c), so q = t* |c|.  Faces whose normal has no z part are never
+    hit (the point-in-triangle test projects along z), so links leaving through the x faces
+    keep -1; so do links to points that are not solid and directions >= ndirs."""
+    tris, planes = _tilted_box((0.37, -3.29, -2.71), (9.61, 3.83, 2.47), 0.3)
+    h = 1.0
+    origin = np.array([-1.0, -5.0, -5.0])
+    dims = (13, 11, 11)
+    k, j, i = np.meshgrid(*[np.arange(n) for n in dims[::-1]], indexing="ij")
+    P = np.stack([origin[0] + i * h, origin[1] + j * h, origin[2] + k * h], -1)
+    sd = np.stack([P @ n - d for n, d in planes], -1)
+    assert np.abs(sd).min() > 1e-3  # no grid point on a face
+    fluid = (sd < 0).all(-1)
+    orc = O.OracleSim(fluid, origin, h, 19, [], tris, 0.8, 8e-6, normals=False)
+    box, q = orc.link_fractions(tris, 19)
+    assert len(box) > 50
+    c, _, _ = O.lattice(19)
+    nx, ny, nz = dims
+    fl = fluid.reshape(-1)
+    hit = 0
+    for w, b in enumerate(box):
+        ii, jj, kk = b % nx, (b // nx) % ny, b // (nx * ny)
+        p = origin + np.array([ii, jj, kk]) * h
+        for d in range(19):
+            a = np.array([ii, jj, kk]) + c[d]
+            if d == 18 or fl[a[0] + nx * (a[1] + ny * a[2])]:
+                assert q[w, d] == -1.0
+                continue
+            cd = c[d].astype(float) * h
+            ts = [((dd - n @ p) / (n @ cd), f) for f, (n, dd) in enumerate(planes) if n @ cd > 0]
+            t, f = min(ts)
+            if f < 2:  # x faces: vertical facets
+                assert q[w, d] == -1.0
+            else:
+                assert q[w, d] == pytest.approx(t * np.linalg.norm(c[d]), abs=1e-12), (b, d)
+                hit += 1
+    assert hit > 100
+    # the same nodes with only the first 6 directions: the diagonal columns stay -1
+    _, q6 = orc.link_fractions(tris, 6)
+    assert (q6[:, 6:] == -1).all() and np.array_equal(q6[:, :6], q[:, :6])
