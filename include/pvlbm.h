@@ -185,13 +185,17 @@ int pvlbm_init_fields(pvlbm_handle h, const double* rho, const double* u);
  * flux_x > 0 the velocity of every plane x is then scaled (by a factor within [1/4, 4]) so that
  * the flux of u_x over the plane's fluid nodes equals flux_x: for a grid whose inlet is the x-min
  * face and whose outlets are all on the x-max face, mass conservation makes the inlet flux the
- * flux of every plane.  Speeds are finally clipped to umax.  Nodes whose sample is not finite
- * start at rest; their number is returned in *nonfinite.  No counterpart in the reference (it
- * always starts from rest, Simulation.cpp:13-15): the start from the learned warm-start network,
- * without a host copy of the full-size field. */
+ * flux of every plane.  plane_flux (host, nx values, may be NULL) gives a target per plane
+ * instead, for a start at an instant where the planes' fluxes differ (the compressible part of a
+ * pulsatile flow); plane_drho (host, nx values, may be NULL) is added to the density of every
+ * node of plane x, which is then clipped again.  Speeds are finally clipped to umax.  Nodes whose
+ * sample is not finite start at rest; their number is returned in *nonfinite.  No counterpart in
+ * the reference (it always starts from rest, Simulation.cpp:13-15): the start from the learned
+ * warm-start network, without a host copy of the full-size field. */
 int pvlbm_init_fields_coarse(pvlbm_handle h, const float* rho_c, const float* u_c,
                              const int32_t cdims[3], double umax, double rho_lo, double rho_hi,
-                             double flux_x, int64_t* nonfinite);
+                             double flux_x, const double* plane_flux, const double* plane_drho,
+                             int64_t* nonfinite);
 
 /* Link fractions of the wall nodes (list PVLBM_LIST_WALL, list order): for each direction
  * d < ndirs whose neighbour is in the box and solid, the segment from the node's point to point +
@@ -235,6 +239,12 @@ int pvlbm_iteration(pvlbm_handle h, int64_t* it);
  * the populations after iteration n-1 (Simulation.cpp:52-105).  Dense box arrays; u holds three
  * planes.  Solid points get 0.  Either pointer may be NULL. */
 int pvlbm_get_macros(pvlbm_handle h, double* rho, double* u);
+
+/* The same fields as FP32 into caller-owned device memory on this handle's device: rho[nbox],
+ * u[3][nbox] (dense box order); points that are not nodes get rho = 1, u = 0, so the arrays can
+ * be sampled as a coarse field by pvlbm_init_fields_coarse of another handle.  No counterpart in
+ * the reference (which has no device memory); used for coarse-grid pre-runs. */
+int pvlbm_get_macros_device(pvlbm_handle h, float* rho, float* u);
 
 /* Per next-to-wall node, in list order: x, y, z, WSS, OSI, TAWSS — the columns of
  * FlowHemodynamics/<iter>_FlowHemodynamics.csv (OutputFile.cpp:867-938), from the last WSS

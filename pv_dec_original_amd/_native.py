@@ -91,7 +91,7 @@ PVLBM_API = [
     ("pvlbm_set_inlet_profile", C.c_int, [C.c_void_p, _d]),
     ("pvlbm_init_fields", C.c_int, [C.c_void_p, _d, _d]),
     ("pvlbm_init_fields_coarse", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_int32),
-                                           C.c_double, C.c_double, C.c_double, C.c_double, _i64]),
+                                           C.c_double, C.c_double, C.c_double, C.c_double, _d, _d, _i64]),
     ("pvlbm_set_wss_every", C.c_int, [C.c_void_p, C.c_int]),
     ("pvlbm_link_fractions", C.c_int, [C.c_void_p, C.c_int, _d]),
     ("pvlbm_set_wall_closure", C.c_int, [C.c_void_p, C.c_int]),
@@ -99,6 +99,7 @@ PVLBM_API = [
     ("pvlbm_step_timed", C.c_int, [C.c_void_p, C.c_int, _d, _d]),
     ("pvlbm_iteration", C.c_int, [C.c_void_p, _i64]),
     ("pvlbm_get_macros", C.c_int, [C.c_void_p, _d, _d]),
+    ("pvlbm_get_macros_device", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     ("pvlbm_get_wss", C.c_int, [C.c_void_p, _d]),
     ("pvlbm_get_wss_raw", C.c_int, [C.c_void_p, _d, _d]),
     ("pvlbm_reduce", C.c_int, [C.c_void_p, _d]),

@@ -834,7 +834,8 @@ extern "C" int pvlbm_init_fields(pvlbm_handle h, const double* rho, const double
 
 extern "C" int pvlbm_init_fields_coarse(pvlbm_handle h, const float* rho_c, const float* u_c,
                                         const int32_t cdims[3], double umax, double rho_lo, double rho_hi,
-                                        double flux_x, int64_t* nonfinite) {
+                                        double flux_x, const double* plane_flux, const double* plane_drho,
+                                        int64_t* nonfinite) {
     if (!h) return PVLBM_EINVAL;
     return guarded(h, [&] {
         if (!rho_c || !u_c || !cdims) fail(PVLBM_EINVAL, "coarse fields and their dims are required");
@@ -844,11 +845,19 @@ extern "C" int pvlbm_init_fields_coarse(pvlbm_handle h, const float* rho_c, cons
         double* d_in = nullptr;
         unsigned long long* d_cnt = nullptr;
         try {
-            d_in = dalloc<double>((size_t)(4 * h->ns + h->nx));
+            // [4][ns] fields, then nx each: plane flux scratch, flux targets, density offsets
+            d_in = dalloc<double>((size_t)(4 * h->ns + 3 * h->nx));
             d_cnt = dalloc<unsigned long long>(1);
+            double* d_tgt = d_in + 4 * h->ns + h->nx;
+            double* d_drho = d_tgt + h->nx;
+            if (plane_flux)
+                HIPCHK(hipMemcpyAsync(d_tgt, plane_flux, sizeof(double) * h->nx, hipMemcpyHostToDevice, h->stream));
+            if (plane_drho)
+                HIPCHK(hipMemcpyAsync(d_drho, plane_drho, sizeof(double) * h->nx, hipMemcpyHostToDevice, h->stream));
             HIPCHK(hipMemsetAsync(d_cnt, 0, sizeof(unsigned long long), h->stream));
             HIPCHK(launch_expand_coarse(rho_c, u_c, cdims[0], cdims[1], cdims[2], h->nx, h->ny, h->nz, h->nxp,
                                         h->nyp, h->ns, umax, rho_lo, rho_hi, h->d_kind, flux_x,
+                                        plane_flux ? d_tgt : nullptr, plane_drho ? d_drho : nullptr,
                                         d_in + 4 * h->ns, d_in, d_cnt, h->stream));
             h->reset_state();
             h->launch(F_INIT | F_MACRO | F_REDUCE, 1.0, d_in, d_in + h->ns);
@@ -1091,6 +1100,16 @@ extern "C" int pvlbm_get_macros(pvlbm_handle h, double* rho, double* u) {
                 if (u)
                     for (int c = 0; c < 3; c++) u[c * nbox + b] = node ? m[(c + 1) * h->ns + n] : 0.0;
             }
+    });
+}
+
+extern "C" int pvlbm_get_macros_device(pvlbm_handle h, float* rho, float* u) {
+    if (!h) return PVLBM_EINVAL;
+    return guarded(h, [&] {
+        if (!rho || !u) fail(PVLBM_EINVAL, "rho and u must be device pointers");
+        HIPCHK(launch_macros_f32(h->d_macro, h->d_kind, h->nx, h->ny, h->nz, h->nxp, h->nyp, h->ns, rho, u,
+                                 h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
     });
 }
 

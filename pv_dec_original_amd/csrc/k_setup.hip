@@ -410,12 +410,14 @@ __global__ __launch_bounds__(256) void k_slab_flux(const double* ux, const uint8
 // velocity of plane x scaled by target / flux[x] (limited to [1/4, 4]; planes without flux keep
 // theirs), then speeds clipped to umax
 __global__ void k_scale_clip(double* out, int64_t ns, int64_t nx, int64_t nxp, const double* flux, double target,
-                             double umax) {
+                             const double* targets, const double* drho, double rlo, double rhi, double umax) {
     const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= ns) return;
     const int64_t x = n % nxp;
+    if (x < nx && targets) target = targets[x];
     double f = 1.0;
     if (flux && target > 0.0 && x < nx && flux[x] > 1e-12 * target) f = fmin(fmax(target / flux[x], 0.25), 4.0);
+    if (drho && x < nx) out[n] = fmin(fmax(out[n] + drho[x], rlo), rhi);
     double u0 = out[ns + n] * f, u1 = out[2 * ns + n] * f, u2 = out[3 * ns + n] * f;
     const double sp = sqrt(u0 * u0 + u1 * u1 + u2 * u2);
     if (sp > umax) {
@@ -431,20 +433,43 @@ __global__ void k_scale_clip(double* out, int64_t ns, int64_t nx, int64_t nxp, c
 
 hipError_t launch_expand_coarse(const float* rho_c, const float* u_c, int cx, int cy, int cz, int64_t nx,
                                 int64_t ny, int64_t nz, int64_t nxp, int64_t nyp, int64_t ns, double umax,
-                                double rlo, double rhi, const uint8_t* kind, double flux_x, double* flux,
-                                double* out, unsigned long long* nonfin, hipStream_t s) {
+                                double rlo, double rhi, const uint8_t* kind, double flux_x, const double* targets,
+                                const double* drho, double* flux, double* out, unsigned long long* nonfin,
+                                hipStream_t s) {
     const int64_t blocks = (ns + 255) / 256;
     hipLaunchKernelGGL(k_expand_coarse, dim3((unsigned)blocks), dim3(256), 0, s, rho_c, u_c, cx, cy, cz, nx, ny,
                        nz, nxp, nyp, ns, rlo, rhi, out, nonfin);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    if (flux_x > 0.0) {
+    const bool scale = flux_x > 0.0 || targets;
+    if (scale) {
         hipLaunchKernelGGL(k_slab_flux, dim3((unsigned)nx), dim3(256), 0, s, out + ns, kind, ny, nz, nxp, nyp, flux);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL(k_scale_clip, dim3((unsigned)blocks), dim3(256), 0, s, out, ns, nx, nxp,
-                       flux_x > 0.0 ? flux : nullptr, flux_x, umax);
+                       scale ? flux : nullptr, flux_x, targets, drho, rlo, rhi, umax);
+    return hipGetLastError();
+}
+
+// one thread per box point; reads the padded slot of the same point
+__global__ void k_macros_f32(const double* macro, const uint8_t* kind, int64_t nx, int64_t ny, int64_t nbox,
+                             int64_t nxp, int64_t nyp, int64_t ns, float* rho, float* u) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbox) return;
+    const int64_t i = b % nx, r = b / nx;
+    const int64_t n = (r % ny + nyp * (r / ny)) * nxp + i;
+    const bool node = kind[n] != 0;
+    rho[b] = node ? (float)macro[n] : 1.0f;
+    for (int c = 0; c < 3; c++) u[c * nbox + b] = node ? (float)macro[(c + 1) * ns + n] : 0.0f;
+}
+
+hipError_t launch_macros_f32(const double* macro, const uint8_t* kind, int64_t nx, int64_t ny, int64_t nz,
+                             int64_t nxp, int64_t nyp, int64_t ns, float* rho, float* u, hipStream_t s) {
+    const int64_t nbox = nx * ny * nz;
+    if (nbox <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_macros_f32, dim3((unsigned)((nbox + 255) / 256)), dim3(256), 0, s, macro, kind, nx, ny,
+                       nbox, nxp, nyp, ns, rho, u);
     return hipGetLastError();
 }
 

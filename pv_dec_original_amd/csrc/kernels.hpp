@@ -113,11 +113,19 @@ hipError_t launch_copy(const void* src, void* dst, int64_t bytes, hipStream_t s)
 // coarse FP32 field (rho_c[cz][cy][cx], u_c[3][cz][cy][cx], voxel centres at fine coordinates
 // (i + 0.5) n / c - 0.5), densities clipped into [rlo, rhi]; padding slots and non-finite samples
 // (counted in *nonfin) get rest.  With flux_x > 0 the velocity of every plane x is scaled so that
-// the flux of u_x over its fluid nodes (kind != 0) is flux_x (flux: nx doubles of scratch);
-// speeds are then clipped to umax.
+// the flux of u_x over its fluid nodes (kind != 0) is flux_x, or targets[x] when targets (nx
+// device doubles) is given (flux: nx doubles of scratch); speeds are then clipped to umax.  With
+// drho (nx device doubles) every slot of plane x gets drho[x] added to its density, clipped again.
 hipError_t launch_expand_coarse(const float* rho_c, const float* u_c, int cx, int cy, int cz, int64_t nx,
                                 int64_t ny, int64_t nz, int64_t nxp, int64_t nyp, int64_t ns, double umax,
-                                double rlo, double rhi, const uint8_t* kind, double flux_x, double* flux,
-                                double* out, unsigned long long* nonfin, hipStream_t s);
+                                double rlo, double rhi, const uint8_t* kind, double flux_x, const double* targets,
+                                const double* drho, double* flux, double* out, unsigned long long* nonfin,
+                                hipStream_t s);
+
+// Compact FP32 box fields from the macro planes (macro[4][ns], padded slots): rho[b] and
+// u[c][b] for every box point b = i + nx (j + ny k); points that are not nodes (kind 0) get
+// rho = 1, u = 0.
+hipError_t launch_macros_f32(const double* macro, const uint8_t* kind, int64_t nx, int64_t ny, int64_t nz,
+                             int64_t nxp, int64_t nyp, int64_t ns, float* rho, float* u, hipStream_t s);
 
 }  // namespace pvlbm

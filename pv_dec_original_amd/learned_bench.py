@@ -69,8 +69,8 @@ def run(weights=DEFAULT_WEIGHTS, box=(1024, 512, 512), tau=0.6, period=4000, sam
     def rest(sim):
         return lambda: (sim.init_fields(), "rest")[1]
 
-    def learned(sim, sd):
-        return lambda: W.start(sim, sd, warm, mean, tau, flux["q"])
+    def learned(sim, sd, phase0=True):
+        return lambda: W.start(sim, sd, warm, mean, tau, flux["q"], (amp, period) if phase0 else None)
 
     # the plain path, continued past its own convergence to a tighter reference record
     plain = CV.run_periods(sim64, "fp64-rest", period, sample_every, mean, amp, max_periods,
@@ -87,6 +87,8 @@ def run(weights=DEFAULT_WEIGHTS, box=(1024, 512, 512), tau=0.6, period=4000, sam
     runs.append(CV.run_periods(sim16, "fp16-warm", period, sample_every, mean, amp, max_periods,
                                stop_change=tol, reference=ref, start=learned(sim16, sd16), log=log))
     if ablations:
+        runs.append(CV.run_periods(sim16, "fp16-warm-mean", period, sample_every, mean, amp, max_periods,
+                                   stop_change=tol, reference=ref, start=learned(sim16, sd16, False), log=log))
         runs.append(CV.run_periods(sim16, "fp16-rest", period, sample_every, mean, amp, max_periods,
                                    stop_change=tol, reference=ref, start=rest(sim16), log=log))
     out["runs"] = {}

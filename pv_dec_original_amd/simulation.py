@@ -51,7 +51,9 @@ REFERENCE_DT = 8e-6
 class SubDomain:
     """Device-resident lattice built from a Geometry (see module doc)."""
 
-    def __init__(self, geometry, faces=(), storage="fp64", tau=1.0, dt=REFERENCE_DT):
+    def __init__(self, geometry, faces=(), storage="fp64", tau=1.0, dt=REFERENCE_DT, triangles=True):
+        """triangles=False creates the lattice without the surface: no next-to-wall normals (WSS
+        reads zero), no link fractions; setup then skips the normals pass over all triangles."""
         self.Geometry_ = geometry
         self.m_velSet = geometry.m_velSet
         self.q = self.m_velSet.q
@@ -64,9 +66,10 @@ class SubDomain:
             g.dims[a] = int(geometry.dims[a])
         h = C.c_void_p()
         fluid = np.ascontiguousarray(geometry.fluid, dtype=np.uint8)
-        tri = np.ascontiguousarray(geometry.globalTriangles, dtype=np.float64)
+        tri = np.ascontiguousarray(geometry.globalTriangles if triangles else np.zeros((0, 12)),
+                                   dtype=np.float64)
         N.check(N.lib().pvlbm_create(N.u8ptr(fluid), C.byref(g), geometry.spacing, self.q,
-                                     STORAGE[storage], arr, len(self.faces), N.dptr(tri), len(tri),
+                                     STORAGE[storage], arr, len(self.faces), N.dptr(tri) if len(tri) else None, len(tri),
                                      float(tau), float(dt), C.byref(h)))
         self.handle = h.value
         self.nx, self.ny, self.nz = geometry.dims
@@ -176,12 +179,13 @@ class Simulation:
             self._chk(N.lib().pvlbm_init_fields(self.handle, N.dptr(r), N.dptr(v)))
         self.total_runtime = 0.0
 
-    def init_fields_coarse(self, rho_c, u_c, umax, rho_range, flux_x=0.0):
+    def init_fields_coarse(self, rho_c, u_c, umax, rho_range, flux_x=0.0, plane_flux=None, plane_drho=None):
         """Restart from a coarse FP32 field held in device memory (torch tensors rho (cz, cy, cx)
         and u (3, cz, cy, cx)), sampled trilinearly at every node, densities clipped into
         rho_range, every x-plane's flux scaled to flux_x when flux_x > 0 (inlet on x-min, outlets
-        on x-max), speeds clipped to umax.  Returns the number of nodes whose sample was not
-        finite (started at rest)."""
+        on x-max) or to plane_flux[x] when that (nx values) is given, plane_drho[x] added to the
+        densities of plane x, speeds clipped to umax.  Returns the number of nodes whose sample was
+        not finite (started at rest)."""
         if rho_c.dtype.itemsize != 4 or u_c.dtype.itemsize != 4 or not (rho_c.is_contiguous() and u_c.is_contiguous()):
             raise ValueError("rho_c and u_c must be contiguous FP32 device tensors")
         cz, cy, cx = rho_c.shape
@@ -189,10 +193,18 @@ class Simulation:
             raise ValueError("u_c must be (3,) + rho_c.shape")
         dims = (C.c_int32 * 3)(cx, cy, cz)
         nf = C.c_int64(0)
+        nx = self.SubDomain_.nx
+        prof = []
+        for a in (plane_flux, plane_drho):
+            if a is not None:
+                a = np.ascontiguousarray(a, dtype=np.float64)
+                if a.shape != (nx,):
+                    raise ValueError("plane profiles need one value per x-plane")
+            prof.append(a)
         self._chk(N.lib().pvlbm_init_fields_coarse(self.handle, C.c_void_p(rho_c.data_ptr()),
                                                    C.c_void_p(u_c.data_ptr()), dims, float(umax),
                                                    float(rho_range[0]), float(rho_range[1]), float(flux_x),
-                                                   C.byref(nf)))
+                                                   N.dptr(prof[0]), N.dptr(prof[1]), C.byref(nf)))
         self.total_runtime = 0.0
         return nf.value
 
@@ -251,6 +263,19 @@ class Simulation:
         rho = np.zeros((sd.nz, sd.ny, sd.nx))
         u = np.zeros((3, sd.nz, sd.ny, sd.nx))
         self._chk(N.lib().pvlbm_get_macros(self.handle, N.dptr(rho), N.dptr(u)))
+        return rho, u
+
+    def macros_device(self):
+        """(rho, u) of the current iteration as FP32 torch tensors on this handle's device,
+        [nz,ny,nx] and [3,nz,ny,nx]; non-node points hold rho = 1, u = 0."""
+        import torch
+        sd = self.SubDomain_
+        dev = torch.device("cuda", torch.cuda.current_device())
+        rho = torch.empty((sd.nz, sd.ny, sd.nx), dtype=torch.float32, device=dev)
+        u = torch.empty((3, sd.nz, sd.ny, sd.nx), dtype=torch.float32, device=dev)
+        torch.cuda.synchronize(dev)
+        self._chk(N.lib().pvlbm_get_macros_device(self.handle, C.c_void_p(rho.data_ptr()),
+                                                  C.c_void_p(u.data_ptr())))
         return rho, u
 
     def wss(self):

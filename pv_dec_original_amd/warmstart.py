@@ -272,22 +272,38 @@ class WarmStart:
         u = torch.where(solid[None], torch.zeros_like(u), u * k)
         return rho, u, cc
 
-    def apply(self, sim, sd, u_mean, tau, flux=None):
+    def apply(self, sim, sd, u_mean, tau, flux=None, pulsatile=None):
         """Start sim from the network's field (True) or leave it untouched when the field fails
         field_ok (False).  When the grid's inlet nodes all lie on the x-min plane and its outlet
         nodes on the x-max plane, the start enforces mass conservation: every x-plane carries the
-        inlet flux `flux` (default u_mean times the number of inlet nodes)."""
+        inlet flux `flux` (default u_mean times the number of inlet nodes).  pulsatile = (amp,
+        period): the inlet runs at (u_mean + amp sin(2 pi t / period)) and the start is its
+        phase 0; the network's steady field is then taken for the period mean and the
+        compressible pulsatile part along x (plane fluxes and a density offset per plane,
+        phase0.phase0_profiles) is imposed on it."""
         kinds = sd.kinds()
         rho, u, cc = self.coarse_fields(kinds, u_mean, tau)
         fl = cc > 0
-        if not field_ok(rho.cpu().numpy(), u.cpu().numpy(), fl):
+        rho_h = rho.cpu().numpy()
+        if not field_ok(rho_h, u.cpu().numpy(), fl):
             return False
         inl, outl = kinds == N.KINLET, kinds == N.KOUTLET
         xfaces = inl.any() and outl.any() and not inl[:, :, 1:].any() and not outl[:, :, :-1].any()
         q = 0.0
+        targets = drho = None
         if xfaces:
             q = float(u_mean * np.count_nonzero(inl[:, :, 0]) if flux is None else flux)
-        nonfin = sim.init_fields_coarse(rho, u, UMAX, RHO_CLIP, q)
+            if pulsatile is not None:
+                from .phase0 import phase0_profiles
+                amp, period = pulsatile
+                # plane-mean density of the network's field, at the fine planes' coarse coordinates
+                nx, cx = kinds.shape[2], rho_h.shape[2]
+                cnt = np.maximum(np.count_nonzero(fl, axis=(0, 1)), 1)
+                rpl = np.where(fl, rho_h, 0.0).sum(axis=(0, 1)) / cnt
+                xc = np.clip((np.arange(nx) + 0.5) * cx / nx - 0.5, 0, cx - 1)
+                targets, drho = phase0_profiles(kinds, q, q * amp / u_mean, period, tau, np.interp(xc, np.arange(cx), rpl))
+                q = 0.0
+        nonfin = sim.init_fields_coarse(rho, u, UMAX, RHO_CLIP, q, targets, drho)
         return nonfin == 0
 
 
@@ -310,13 +326,13 @@ def clip_field(rho, u):
     return np.clip(rho, *RHO_CLIP), u * f[None]
 
 
-def start(sim, sd, warm, u_mean, tau, flux=None):
+def start(sim, sd, warm, u_mean, tau, flux=None, pulsatile=None):
     """Initialise sim from the network's field when it passes field_ok, else from rest.
-    Returns 'warm' or 'rest'.  flux: the inlet flux (sum of the inlet velocities), see
-    WarmStart.apply."""
+    Returns 'warm' or 'rest'.  flux: the inlet flux (sum of the inlet velocities); pulsatile:
+    (amp, period) for a start at phase 0 of a pulsatile inlet; see WarmStart.apply."""
     if warm is not None:
         try:
-            if warm.apply(sim, sd, u_mean, tau, flux):
+            if warm.apply(sim, sd, u_mean, tau, flux, pulsatile):
                 return "warm"
         except (ValueError, N.NativeError):
             pass

@@ -28,7 +28,7 @@ class _Net:
     def __init__(self, rho, u, fail=None):
         self.rho, self.u, self.fail = rho, u, fail
 
-    def apply(self, sim, sd, u_mean, tau, flux=None):
+    def apply(self, sim, sd, u_mean, tau, flux=None, pulsatile=None):
         if self.fail:
             raise self.fail
         if not W.field_ok(self.rho, self.u, sd.kinds() != N.SOLID):

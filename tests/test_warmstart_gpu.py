@@ -111,3 +111,48 @@ def test_trained_network_shortens_the_run_to_converged_wss(W):
     ratio = sum(warm_steps) / sum(rest_steps)
     print(f"steps to 1% WSS: rest {rest_steps} warm {warm_steps} ratio {ratio:.3f}")
     assert ratio <= 0.8, (rest_steps, warm_steps)
+
+
+def test_phase0_profiles_remove_the_acoustic_transient(W):
+    """Pulsatile flow in a wide tube (R = 16, 512 long, tau 0.6, period 2000 steps: Womersley
+    number 4.9, 0.44 acoustic wavelengths, the proportions of the large aneurysm).  Both starts
+    take the converged steady field at the mean inlet velocity; imposing the phase-0 plane fluxes
+    and densities of the one-dimensional model (phase0.phase0_profiles, through
+    pvlbm_init_fields_coarse) must leave a several times smaller WSS error against the periodic
+    state in the following periods than the steady field alone, whose density misses the
+    oscillation's acoustic part and rings (measured: 0.068 / 0.021 against 0.011 / 0.0046 in
+    periods 2 and 3)."""
+    import torch
+    from pv_dec_original_amd import converge as CV
+    from pv_dec_original_amd.phase0 import phase0_profiles
+    from pv_dec_original_amd.simulation import SubDomain, Simulation
+    from pv_dec_original_amd.workloads import tube_case
+    T, se, mean, amp, tau = 2000, 100, 0.03, 0.02, 0.6
+    case = tube_case(q=19, radius=16.0, length=512.0, tau=tau)
+    sd = SubDomain(case.geometry, case.faces, storage="fp64", tau=tau, dt=8e-6)
+    sim = Simulation(sd, tau, 1.0, 8e-6)
+    prof = case.inlet_profile(sd, 2.0)
+    prof = prof / prof.mean()
+    sim.set_inlet_profile(prof)
+    q = mean * float(prof.sum())
+    sim.step(24000, np.full(24000, mean))
+    rho_s, u_s = sim.macros_device()
+    ref = CV.run_periods(sim, "ref", T, se, mean, amp, 30, stop_change=2e-4)
+    assert ref.history[-1]["change"] < 2e-4
+    rec = ref.records[-1]
+    kinds = sd.kinds()
+    fl = kinds != 0
+    rpl = np.where(fl, rho_s.cpu().numpy(), 0.0).sum(axis=(0, 1)) / np.maximum(fl.sum(axis=(0, 1)), 1)
+    targets, drho = phase0_profiles(kinds, q, q * amp / mean, T, tau, rpl)
+    err = {}
+    for name, prof_args in (("steady", (q, None, None)), ("phase0", (0.0, targets, drho))):
+        def start(a=prof_args):
+            assert sim.init_fields_coarse(rho_s, u_s, W.UMAX, W.RHO_CLIP, *a) == 0
+            return name
+        r = CV.run_periods(sim, name, T, se, mean, amp, 3, stop_change=0.0, reference=rec, start=start)
+        err[name] = [h["error"] for h in r.history]
+    print(f"WSS error per period against the periodic state: steady start {np.round(err['steady'], 4)}, "
+          f"with phase-0 profiles {np.round(err['phase0'], 4)}")
+    assert err["phase0"][0] < err["steady"][0]
+    assert err["phase0"][1] < 0.35 * err["steady"][1] and err["phase0"][2] < 0.5 * err["steady"][2]
+    torch.cuda.synchronize()
