@@ -165,6 +165,12 @@ int pvlbm_list(pvlbm_handle h, int list, int64_t* idx);
 /* Node kind of every box point (enum pvlbm_kind), nx*ny*nz bytes. */
 int pvlbm_get_kinds(pvlbm_handle h, uint8_t* kinds);
 
+/* Node kinds on a separable grid of box points: out[(k ny + j) nx + i] = kind of point
+ * (ix[i], iy[j], iz[k]); every index must lie in the box.  No counterpart in the reference: a
+ * coarse view of the kinds (the warm-start network's input) without copying the whole box. */
+int pvlbm_sample_kinds(pvlbm_handle h, const int32_t* ix, int32_t nx, const int32_t* iy, int32_t ny,
+                       const int32_t* iz, int32_t nz, uint8_t* out);
+
 /* Inlet velocity of each inlet-list node; the x-velocity applied at step s is
  * inlet_scale[s] * u[node].  Default 1.0 everywhere, which makes inlet_scale the reference's
  * uniform InletVelocity_ (Simulation.cpp:580, Boundary.cpp:131-133). */
@@ -250,6 +256,11 @@ int pvlbm_get_macros_device(pvlbm_handle h, float* rho, float* u);
  * FlowHemodynamics/<iter>_FlowHemodynamics.csv (OutputFile.cpp:867-938), from the last WSS
  * evaluation (Simulation.cpp:880-905). */
 int pvlbm_get_wss(pvlbm_handle h, double* out6);
+
+/* The WSS column of pvlbm_get_wss alone (nw doubles in next-to-wall list order: the magnitude of
+ * Simulation.cpp:880-898 from the last WSS evaluation), without coordinates, OSI and TAWSS, for
+ * callers that sample WSS often. */
+int pvlbm_get_wss_magnitude(pvlbm_handle h, double* out);
 
 /* Raw WSS state per next-to-wall node: tau[3][nw] of the last evaluation and the accumulators
  * acc[4][nw] = (sum tau_x, sum tau_y, sum tau_z, sum |tau|). Either pointer may be NULL. */

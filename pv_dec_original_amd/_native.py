@@ -88,6 +88,8 @@ PVLBM_API = [
     ("pvlbm_count", C.c_int, [C.c_void_p, C.c_int, _i64]),
     ("pvlbm_list", C.c_int, [C.c_void_p, C.c_int, _i64]),
     ("pvlbm_get_kinds", C.c_int, [C.c_void_p, _u8]),
+    ("pvlbm_sample_kinds", C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.c_int32, C.POINTER(C.c_int32), C.c_int32,
+                                     C.POINTER(C.c_int32), C.c_int32, _u8]),
     ("pvlbm_set_inlet_profile", C.c_int, [C.c_void_p, _d]),
     ("pvlbm_init_fields", C.c_int, [C.c_void_p, _d, _d]),
     ("pvlbm_init_fields_coarse", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_int32),
@@ -101,6 +103,7 @@ PVLBM_API = [
     ("pvlbm_get_macros", C.c_int, [C.c_void_p, _d, _d]),
     ("pvlbm_get_macros_device", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     ("pvlbm_get_wss", C.c_int, [C.c_void_p, _d]),
+    ("pvlbm_get_wss_magnitude", C.c_int, [C.c_void_p, _d]),
     ("pvlbm_get_wss_raw", C.c_int, [C.c_void_p, _d, _d]),
     ("pvlbm_reduce", C.c_int, [C.c_void_p, _d]),
     ("pvlbm_gate_events", C.c_int, [C.c_void_p, _i64]),

@@ -67,7 +67,7 @@ def run_periods(sim, label, period=4000, sample_every=200, mean=0.03, amp=0.02, 
             ms = sim.step(sample_every, pulsatile_scales(it, sample_every, period, mean, amp), timed=True)
             res.kernel_ms += ms
             it += sample_every
-            rec.append(sim.wss()[:, 3].copy())
+            rec.append(sim.wss_magnitude())
         rec = np.array(rec)
         res.periods = P
         res.wall_s = time.perf_counter() - t0

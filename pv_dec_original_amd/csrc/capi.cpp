@@ -788,6 +788,41 @@ extern "C" int pvlbm_get_kinds(pvlbm_handle h, uint8_t* kinds) {
     });
 }
 
+extern "C" int pvlbm_sample_kinds(pvlbm_handle h, const int32_t* ix, int32_t nx, const int32_t* iy, int32_t ny,
+                                  const int32_t* iz, int32_t nz, uint8_t* out) {
+    if (!h) return PVLBM_EINVAL;
+    return guarded(h, [&] {
+        if (!ix || !iy || !iz || !out || nx < 1 || ny < 1 || nz < 1) fail(PVLBM_EINVAL, "bad sample grid");
+        const int32_t* idx[3] = {ix, iy, iz};
+        const int32_t cnt[3] = {nx, ny, nz};
+        const int64_t dims[3] = {h->nx, h->ny, h->nz};
+        for (int a = 0; a < 3; a++)
+            for (int32_t i = 0; i < cnt[a]; i++)
+                if (idx[a][i] < 0 || idx[a][i] >= dims[a]) fail(PVLBM_EINVAL, "sample index outside the box");
+        const size_t n = (size_t)nx * ny * nz;
+        int32_t* d_idx = nullptr;
+        uint8_t* d_out = nullptr;
+        try {
+            d_idx = dalloc<int32_t>((size_t)nx + ny + nz);
+            d_out = dalloc<uint8_t>(n);
+            HIPCHK(hipMemcpyAsync(d_idx, ix, sizeof(int32_t) * nx, hipMemcpyHostToDevice, h->stream));
+            HIPCHK(hipMemcpyAsync(d_idx + nx, iy, sizeof(int32_t) * ny, hipMemcpyHostToDevice, h->stream));
+            HIPCHK(hipMemcpyAsync(d_idx + nx + ny, iz, sizeof(int32_t) * nz, hipMemcpyHostToDevice, h->stream));
+            HIPCHK(launch_sample_kinds(h->d_kind, h->nxp, h->nyp, d_idx, nx, d_idx + nx, ny, d_idx + nx + ny, nz,
+                                       d_out, h->stream));
+            HIPCHK(hipMemcpyAsync(out, d_out, n, hipMemcpyDeviceToHost, h->stream));
+            HIPCHK(hipStreamSynchronize(h->stream));
+        } catch (...) {
+            (void)hipStreamSynchronize(h->stream);
+            dfree(d_idx);
+            dfree(d_out);
+            throw;
+        }
+        dfree(d_idx);
+        dfree(d_out);
+    });
+}
+
 extern "C" int pvlbm_set_inlet_profile(pvlbm_handle h, const double* u) {
     if (!h) return PVLBM_EINVAL;
     return guarded(h, [&] {
@@ -1110,6 +1145,17 @@ extern "C" int pvlbm_get_macros_device(pvlbm_handle h, float* rho, float* u) {
         HIPCHK(launch_macros_f32(h->d_macro, h->d_kind, h->nx, h->ny, h->nz, h->nxp, h->nyp, h->ns, rho, u,
                                  h->stream));
         HIPCHK(hipStreamSynchronize(h->stream));
+    });
+}
+
+extern "C" int pvlbm_get_wss_magnitude(pvlbm_handle h, double* out) {
+    if (!h) return PVLBM_EINVAL;
+    return guarded(h, [&] {
+        if (!out) fail(PVLBM_EINVAL, "out is NULL");
+        if (h->nw > 0) {
+            HIPCHK(hipMemcpyAsync(out, h->d_wss, sizeof(double) * h->nw, hipMemcpyDeviceToHost, h->stream));
+            HIPCHK(hipStreamSynchronize(h->stream));
+        }
     });
 }
 

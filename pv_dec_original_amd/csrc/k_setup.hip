@@ -473,6 +473,26 @@ hipError_t launch_macros_f32(const double* macro, const uint8_t* kind, int64_t n
     return hipGetLastError();
 }
 
+// one thread per output point of a separable index grid
+__global__ void k_sample_kinds(const uint8_t* kind, int64_t nxp, int64_t nyp, const int32_t* ix, int nx,
+                               const int32_t* iy, int ny, const int32_t* iz, int nz, uint8_t* out) {
+    const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= (int64_t)nx * ny * nz) return;
+    const int i = (int)(o % nx);
+    const int64_t r = o / nx;
+    const int j = (int)(r % ny), k = (int)(r / ny);
+    out[o] = kind[ix[i] + nxp * (iy[j] + nyp * (int64_t)iz[k])];
+}
+
+hipError_t launch_sample_kinds(const uint8_t* kind, int64_t nxp, int64_t nyp, const int32_t* ix, int nx,
+                               const int32_t* iy, int ny, const int32_t* iz, int nz, uint8_t* out, hipStream_t s) {
+    const int64_t n = (int64_t)nx * ny * nz;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sample_kinds, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, kind, nxp, nyp, ix, nx,
+                       iy, ny, iz, nz, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_copy(const void* src, void* dst, int64_t bytes, hipStream_t s) {
     const int64_t n = bytes / 16;
     hipLaunchKernelGGL(k_copy, dim3(8192), dim3(256), 0, s, static_cast<const double2*>(src),

@@ -128,4 +128,9 @@ hipError_t launch_expand_coarse(const float* rho_c, const float* u_c, int cx, in
 hipError_t launch_macros_f32(const double* macro, const uint8_t* kind, int64_t nx, int64_t ny, int64_t nz,
                              int64_t nxp, int64_t nyp, int64_t ns, float* rho, float* u, hipStream_t s);
 
+// out[(k ny + j) nx + i] = kind of box point (ix[i], iy[j], iz[k]) (padded layout nxp x nyp);
+// the indices must lie in the box (device arrays).
+hipError_t launch_sample_kinds(const uint8_t* kind, int64_t nxp, int64_t nyp, const int32_t* ix, int nx,
+                               const int32_t* iy, int ny, const int32_t* iz, int nz, uint8_t* out, hipStream_t s);
+
 }  // namespace pvlbm

@@ -43,7 +43,10 @@ def run(weights=DEFAULT_WEIGHTS, box=(1024, 512, 512), tau=0.6, period=4000, sam
     t0 = time.perf_counter()
     case = aneurysm_case(q=19, box=box, tau=tau)
     log(f"aneurysm geometry: {len(case.geometry.globalTriangles)} triangles, {time.perf_counter() - t0:.1f} s")
+    t = time.perf_counter()
     warm = W.WarmStart(weights)
+    warm.prepare(box[::-1])
+    net_setup_s = time.perf_counter() - t
 
     flux = {}
 
@@ -62,6 +65,7 @@ def run(weights=DEFAULT_WEIGHTS, box=(1024, 512, 512), tau=0.6, period=4000, sam
            "reference": f"FP64 from rest until the period-to-period WSS change < {ref_change}",
            "criterion": f"each path stops when its period-to-period WSS change < {tol}; the chosen "
                         f"path's record then must be within {tol} (L2) of the reference"}
+    out["net_setup_s"] = round(net_setup_s, 3)  # weights read and buffers allocated, before any run
     sd64, sim64 = make("fp64")
     out["fluid_nodes"] = int(sd64.counts()["fluid"])
     out["ntw_nodes"] = int(sd64.counts()["ntw"])

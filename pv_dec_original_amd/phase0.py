@@ -49,6 +49,13 @@ def plane_areas(kinds):
     return nodes - 0.5 * np.count_nonzero(k == WALL, axis=(0, 1))
 
 
+def plane_areas_from_lists(nx, nodes, walls):
+    """plane_areas from box indices (x fastest) of all nodes and of the wall nodes: the same
+    counts without a pass over the whole box."""
+    n = np.bincount(np.asarray(nodes) % nx, minlength=nx).astype(np.float64)
+    return n - 0.5 * np.bincount(np.asarray(walls) % nx, minlength=nx)
+
+
 def oscillation(area, q_amp, period, tau):
     """Complex amplitudes (q_hat, rho_hat) per plane of the pulsatile part, for an inlet flux
     Q_mean + q_amp sin(2 pi t / period) on plane 0 and rho = rho_out on the last plane."""
@@ -77,12 +84,13 @@ def oscillation(area, q_amp, period, tau):
     return q_hat, p_hat / CS2
 
 
-def phase0_profiles(kinds, q_mean, q_amp, period, tau, rho_mean_planes=None):
-    """Per-plane targets at phase 0 (inlet flux rising through its mean): the flux of u_x over
-    each plane and the density offset Re rho_hat to add to a period-mean density field.  The
-    flux is the mean part, which carries the inlet's mass flux q_mean rho(0) through planes of
-    period-mean density rho_mean_planes (default 1), plus the pulsatile part Re q_hat; on the
-    inlet plane it is q_mean, the flux the inlet imposes at phase 0."""
-    q_hat, r_hat = oscillation(plane_areas(kinds), q_amp, period, tau)
+def phase0_profiles(area, q_mean, q_amp, period, tau, rho_mean_planes=None):
+    """Per-plane targets at phase 0 (inlet flux rising through its mean) for planes of effective
+    cross-section `area` (plane_areas): the flux of u_x over each plane and the density offset
+    Re rho_hat to add to a period-mean density field.  The flux is the mean part, which carries
+    the inlet's mass flux q_mean rho(0) through planes of period-mean density rho_mean_planes
+    (default 1), plus the pulsatile part Re q_hat; on the inlet plane it is q_mean, the flux the
+    inlet imposes at phase 0."""
+    q_hat, r_hat = oscillation(area, q_amp, period, tau)
     rpl = np.ones(len(q_hat)) if rho_mean_planes is None else np.asarray(rho_mean_planes, dtype=np.float64)
     return q_mean * rpl[0] / rpl + q_hat.real, r_hat.real

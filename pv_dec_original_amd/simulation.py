@@ -124,6 +124,16 @@ class SubDomain:
         self._chk(N.lib().pvlbm_get_kinds(self.handle, N.u8ptr(k)))
         return k
 
+    def sample_kinds(self, ix, iy, iz):
+        """Kinds at the box points (ix[i], iy[j], iz[k]) as an array [len(iz), len(iy), len(ix)],
+        gathered on the device."""
+        ax = [np.ascontiguousarray(a, dtype=np.int32) for a in (ix, iy, iz)]
+        out = np.zeros((len(ax[2]), len(ax[1]), len(ax[0])), dtype=np.uint8)
+        p = [a.ctypes.data_as(C.POINTER(C.c_int32)) for a in ax]
+        self._chk(N.lib().pvlbm_sample_kinds(self.handle, p[0], len(ax[0]), p[1], len(ax[1]), p[2], len(ax[2]),
+                                             N.u8ptr(out)))
+        return out
+
     def counts(self):
         return {name: self.count(lid) for name, lid in
                 [("fluid", N.LIST_FLUID), ("inlet", N.LIST_INLET), ("outlet1", N.LIST_OUTLET1),
@@ -284,6 +294,13 @@ class Simulation:
         out = np.zeros((nw, 6))
         if nw:
             self._chk(N.lib().pvlbm_get_wss(self.handle, N.dptr(out)))
+        return out
+
+    def wss_magnitude(self):
+        """[nw]: the WSS column of wss() alone (one device copy of nw doubles)."""
+        out = np.zeros(self.SubDomain_.count(N.LIST_NTW))
+        if len(out):
+            self._chk(N.lib().pvlbm_get_wss_magnitude(self.handle, N.dptr(out)))
         return out
 
     def wss_raw(self):

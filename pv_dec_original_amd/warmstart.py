@@ -205,6 +205,25 @@ def coarse_code(kinds, cshape):
     return c
 
 
+def coarse_code_subdomain(sd, cshape):
+    """coarse_code(sd.kinds(), cshape) without copying the kinds of the whole box: the coarse
+    voxel centres are gathered on the device, and the inlet / outlet faces are read from the
+    node lists."""
+    shape = (sd.nz, sd.ny, sd.nx)
+    if tuple(cshape) == shape:
+        return code_from_kinds(sd.kinds())
+    iz, iy, ix = [np.minimum(((np.arange(c) + 0.5) * n / c).astype(np.int64), n - 1) for c, n in zip(cshape, shape)]
+    c = (sd.sample_kinds(ix, iy, iz) != N.SOLID).astype(np.uint8)
+    fl = c > 0
+    nx = sd.nx
+    if (sd.list(N.LIST_INLET) % nx == 0).any():
+        c[:, :, 0][fl[:, :, 0]] = 2
+    outl = np.concatenate([sd.list(l) for l in (N.LIST_OUTLET1, N.LIST_OUTLET2, N.LIST_OUTLET3)])
+    if (outl % nx == nx - 1).any():
+        c[:, :, -1][fl[:, :, -1]] = 3
+    return c
+
+
 def _trilinear_up(field, shape):
     """Values at the fine nodes of `shape` from values at the centres of the coarse voxels of
     `field` (coarse voxel k spans fine coordinates [k, k+1) * n/c); clamped at the edges.  The
@@ -255,12 +274,19 @@ class WarmStart:
         rho = 1.0 + y[..., 3].contiguous() * u_mean
         return rho, u
 
-    def coarse_fields(self, kinds, u_mean, tau):
+    def prepare(self, shape):
+        """Allocate the network's buffers for a solver grid of shape (nz, ny, nx) ahead of the
+        first start (part of loading the model, like reading its weights)."""
+        cshape, _ = self.net_grid(shape)
+        self.net.buffers((1,) + tuple(cshape))
+
+    def coarse_fields(self, kinds, u_mean, tau, sd=None):
         """The network's field for a solver grid, on the network grid: device tensors rho
         (cz, cy, cx) and u (3, cz, cy, cx) for the solver's (u_mean, tau), rest on solid voxels,
-        plus the coarse code."""
-        cshape, s = self.net_grid(kinds.shape)
-        cc = coarse_code(kinds, cshape)
+        plus the coarse code.  kinds: the grid's node kinds, or None with sd given (the code is
+        then gathered on the device, coarse_code_subdomain)."""
+        cshape, s = self.net_grid(kinds.shape if kinds is not None else (sd.nz, sd.ny, sd.nx))
+        cc = coarse_code(kinds, cshape) if kinds is not None else coarse_code_subdomain(sd, cshape)
         uc, tc = coarse_params(u_mean, tau, s) if s > 1 else (u_mean, tau)
         rho, u = self.predict(cc, uc, tc)
         # at equal Reynolds number u / u_mean is the same field, and rho - 1 (viscous drop
@@ -281,27 +307,31 @@ class WarmStart:
         phase 0; the network's steady field is then taken for the period mean and the
         compressible pulsatile part along x (plane fluxes and a density offset per plane,
         phase0.phase0_profiles) is imposed on it."""
-        kinds = sd.kinds()
-        rho, u, cc = self.coarse_fields(kinds, u_mean, tau)
+        rho, u, cc = self.coarse_fields(None, u_mean, tau, sd)
         fl = cc > 0
         rho_h = rho.cpu().numpy()
         if not field_ok(rho_h, u.cpu().numpy(), fl):
             return False
-        inl, outl = kinds == N.KINLET, kinds == N.KOUTLET
-        xfaces = inl.any() and outl.any() and not inl[:, :, 1:].any() and not outl[:, :, :-1].any()
+        # inlet nodes all on the x-min plane, outlet nodes all on the x-max plane (from the lists)
+        nx = sd.nx
+        inl = sd.list(N.LIST_INLET) % nx
+        outl = np.concatenate([sd.list(l) for l in (N.LIST_OUTLET1, N.LIST_OUTLET2, N.LIST_OUTLET3)]) % nx
+        xfaces = len(inl) > 0 and len(outl) > 0 and not inl.any() and (outl == nx - 1).all()
         q = 0.0
         targets = drho = None
         if xfaces:
-            q = float(u_mean * np.count_nonzero(inl[:, :, 0]) if flux is None else flux)
+            q = float(u_mean * len(inl) if flux is None else flux)
             if pulsatile is not None:
-                from .phase0 import phase0_profiles
+                from .phase0 import phase0_profiles, plane_areas_from_lists
                 amp, period = pulsatile
                 # plane-mean density of the network's field, at the fine planes' coarse coordinates
-                nx, cx = kinds.shape[2], rho_h.shape[2]
+                cx = rho_h.shape[2]
                 cnt = np.maximum(np.count_nonzero(fl, axis=(0, 1)), 1)
                 rpl = np.where(fl, rho_h, 0.0).sum(axis=(0, 1)) / cnt
                 xc = np.clip((np.arange(nx) + 0.5) * cx / nx - 0.5, 0, cx - 1)
-                targets, drho = phase0_profiles(kinds, q, q * amp / u_mean, period, tau, np.interp(xc, np.arange(cx), rpl))
+                area = plane_areas_from_lists(nx, sd.list(N.LIST_FLUID), sd.list(N.LIST_WALL))
+                targets, drho = phase0_profiles(area, q, q * amp / u_mean, period, tau,
+                                                np.interp(xc, np.arange(cx), rpl))
                 q = 0.0
         nonfin = sim.init_fields_coarse(rho, u, UMAX, RHO_CLIP, q, targets, drho)
         return nonfin == 0

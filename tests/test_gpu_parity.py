@@ -134,6 +134,7 @@ def test_wss_sampling_matches_oracle():
     w = sim.wss()
     wo = orc.wss()[0]
     assert G.rel_err(w[:, 3], wo[:, 3]) <= TOL and G.rel_err(w[:, 5], wo[:, 5]) <= TOL
+    assert np.array_equal(sim.wss_magnitude(), w[:, 3])
 
 
 def test_fp64_global_sums():

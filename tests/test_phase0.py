@@ -61,9 +61,14 @@ def test_short_tube_inertial_limit():
 
 def test_phase0_profiles_shapes():
     kinds = np.ones((5, 6, 40), dtype=np.uint8)
-    t, d = P.phase0_profiles(kinds, 3.0, 2.0, 500, 0.6)
+    kinds[0] = 5  # a wall layer
+    area = P.plane_areas(kinds)
+    assert np.array_equal(area, np.full(40, 30.0 - 0.5 * 6))
+    box = np.flatnonzero(kinds.reshape(-1))
+    assert np.array_equal(P.plane_areas_from_lists(40, box, np.flatnonzero(kinds.reshape(-1) == 5)), area)
+    t, d = P.phase0_profiles(area, 3.0, 2.0, 500, 0.6)
     assert t.shape == d.shape == (40,) and abs(d[-1]) < 1e-15 and abs(t[0] - 3.0) < 1e-12
     # the mean part keeps the mass flux: q rho(0) / rho(x) through denser or lighter planes
     rpl = np.linspace(1.02, 1.0, 40)
-    t2, d2 = P.phase0_profiles(kinds, 3.0, 2.0, 500, 0.6, rpl)
+    t2, d2 = P.phase0_profiles(area, 3.0, 2.0, 500, 0.6, rpl)
     assert np.allclose(t2 - t, 3.0 * (rpl[0] / rpl - 1.0), atol=1e-12) and np.array_equal(d2, d)

@@ -124,7 +124,8 @@ def test_phase0_profiles_remove_the_acoustic_transient(W):
     periods 2 and 3)."""
     import torch
     from pv_dec_original_amd import converge as CV
-    from pv_dec_original_amd.phase0 import phase0_profiles
+    from pv_dec_original_amd import _native as N
+    from pv_dec_original_amd.phase0 import phase0_profiles, plane_areas, plane_areas_from_lists
     from pv_dec_original_amd.simulation import SubDomain, Simulation
     from pv_dec_original_amd.workloads import tube_case
     T, se, mean, amp, tau = 2000, 100, 0.03, 0.02, 0.6
@@ -143,7 +144,9 @@ def test_phase0_profiles_remove_the_acoustic_transient(W):
     kinds = sd.kinds()
     fl = kinds != 0
     rpl = np.where(fl, rho_s.cpu().numpy(), 0.0).sum(axis=(0, 1)) / np.maximum(fl.sum(axis=(0, 1)), 1)
-    targets, drho = phase0_profiles(kinds, q, q * amp / mean, T, tau, rpl)
+    area = plane_areas(kinds)
+    assert np.array_equal(area, plane_areas_from_lists(kinds.shape[2], sd.list(N.LIST_FLUID), sd.list(N.LIST_WALL)))
+    targets, drho = phase0_profiles(area, q, q * amp / mean, T, tau, rpl)
     err = {}
     for name, prof_args in (("steady", (q, None, None)), ("phase0", (0.0, targets, drho))):
         def start(a=prof_args):
@@ -156,3 +159,15 @@ def test_phase0_profiles_remove_the_acoustic_transient(W):
     assert err["phase0"][0] < err["steady"][0]
     assert err["phase0"][1] < 0.35 * err["steady"][1] and err["phase0"][2] < 0.5 * err["steady"][2]
     torch.cuda.synchronize()
+
+
+def test_coarse_code_gathered_on_the_device(W):
+    """coarse_code_subdomain (device gather of the coarse voxel centres, faces from the lists)
+    equals coarse_code over the host copy of the whole box."""
+    from pv_dec_original_amd.simulation import SubDomain
+    from pv_dec_original_amd.workloads import vessel_case
+    case = vessel_case(VAL_SEED0 + 3)
+    sd = SubDomain(case.geometry, case.faces, storage="fp16", tau=case.tau)
+    kinds = sd.kinds()
+    for cshape in ((48, 48, 48), (40, 56, 64), kinds.shape):
+        assert np.array_equal(W.coarse_code_subdomain(sd, cshape), W.coarse_code(kinds, cshape)), cshape
