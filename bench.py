@@ -167,7 +167,7 @@ def main():
         copy_gbs = 2 * (args.copy_bytes // 16 * 16) * 10 / (cms.value / 1e3) / 1e9
         log(f"copy kernel: {copy_gbs:.0f} GB/s (read + write of {args.copy_bytes} bytes)")
 
-    cpu = None
+    cpu = cpu_threads = None
     if not args.no_cpu:
         k = sd.kinds().reshape(-1)
         pbox = np.zeros(k.size)
@@ -178,6 +178,13 @@ def main():
                          f"-O2 -ffp-contract=off, 1 thread) in {el:.1f} s on {cpu_model()}; "
                          "normals skipped (timing only)"}
         log(f"cpu baseline {v:.2f} MLUPS ({n} steps)")
+        # the same oracle over the host cores this process may use (a second, threaded baseline)
+        nthr = max(1, min(len(os.sched_getaffinity(0)), int(os.environ.get("OMP_NUM_THREADS", "16") or 16)))
+        vt, nt, elt = cpu_baseline(case, pbox, q, tau, args.cpu_budget / 2, nthr)
+        cpu_threads = {"value": round(vt, 3), "unit": "MLUPS", "cores": nthr, "kind": "port",
+                       "sample": f"{nt} iterations by the CPU oracle with OpenMP over {nthr} threads in "
+                                 f"{elt:.1f} s on {cpu_model()}; normals skipped (timing only)"}
+        log(f"cpu baseline, {nthr} threads: {vt:.2f} MLUPS ({nt} steps)")
 
     workload = f"{args.workload}-D3Q{q}-{args.storage}-{box[0]}x{box[1]}x{box[2]}"
     out = {
@@ -209,6 +216,7 @@ def main():
             "copy_kernel_GBs": None if copy_gbs is None else round(copy_gbs, 1),
         },
         "cpu_baseline": cpu,
+        "cpu_baseline_threads": cpu_threads,
     }
     if args.learned:
         del sd, sim  # free the Y-bifurcation's device memory first
