@@ -3,7 +3,7 @@
  * The reference has no counterpart: its solver always starts from rest (Simulation.cpp:13-15 sets
  * every population to its weight before the time loop).  These entry points are the compute of a
  * small 3D U-Net that maps voxel geometry plus inlet parameters to an initial (u, rho) field; the
- * solver of pvlbm.h runs on from that field (pvlbm_init_fields) and remains the judge.
+ * solver of pvlbm.h runs on from that field (pvlbm_init_fields) and decides the result.
  *
  * Tensors are batches of boxes, channels-last: element (b, z, y, x, c) of a tensor with row
  * length ld sits at ((((b*d + z)*h + y)*w + x)*ld + c).  Activations and activation gradients are

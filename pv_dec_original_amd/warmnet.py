@@ -5,7 +5,7 @@ weight).  This network maps a geometry code volume (0 solid, 1 fluid, 2 inlet fa
 its wall distance and the inlet parameters (u_mean, tau, inlet area) to an initial field,
 normalised as
     y[..., 0:3] = u / u_mean,   y[..., 3] = (rho - 1) / u_mean,
-from which the solver runs on (pvlbm_init_fields); the solver judges the field and starting from
+from which the solver runs on (pvlbm_init_fields); the solver decides the result, and starting from
 rest stays the fallback.
 
 Architecture (levels = channels per resolution, default (16, 32, 64, 160), 1.82 M parameters):

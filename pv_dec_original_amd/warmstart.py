@@ -1,4 +1,4 @@
-"""Learned warm start: solver-generated training data, training, inference and the judged start.
+"""Learned warm start: solver-generated training data, training, inference and the start.
 
 Training data (one sample per seed): a seeded tortuous vessel in a 96^3 box (workloads.vessel_mesh:
 cubic-spline centre line, radius 6-16, 1-2 branches, straight flow extensions at the x faces),
@@ -174,7 +174,7 @@ def evaluate(net, codes, targets, prms, batch=4):
 
 
 # ---------------------------------------------------------------------------------------------
-# inference and the judged warm start
+# inference and the warm start
 
 def coarse_params(u_mean, tau, s):
     """Network inputs (u_c, tau_c) on a grid coarsened by s with the solver's Reynolds number
@@ -371,7 +371,7 @@ def start(sim, sd, warm, u_mean, tau, flux=None, pulsatile=None):
 
 
 # ---------------------------------------------------------------------------------------------
-# convergence of wall shear stress: the measure the warm start is judged by
+# convergence of wall shear stress: the measure of the warm start's gain
 
 def reference_wss(sim, fluid, tol=1e-6, max_steps=40000):
     """WSS magnitude per next-to-wall node of the converged steady state (run from rest with the
