@@ -48,7 +48,8 @@ def measured_traffic(workload):
     """HBM bytes per launch of the step kernel from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
     summary of the same workload (calibrated against a copy kernel), or None."""
     name = {"ybif-D3Q27-fp64-512x256x256": "k_step_ybif512_q27_fp64.json",
-            "ybif-D3Q19-fp16-512x256x256": "k_step_ybif512_q19_fp16.json"}.get(workload)
+            "ybif-D3Q19-fp16-512x256x256": "k_step_ybif512_q19_fp16.json",
+            "aneurysm-D3Q19-fp16-1024x512x512": "k_step_aneurysm1024_q19_fp16.json"}.get(workload)
     if not name:
         return None
     path = os.path.join(ROOT, "profiles", name)
@@ -88,9 +89,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--lattice", type=int, default=27)
     ap.add_argument("--storage", default="fp64", choices=["fp64", "fp16", "bf16"])
-    ap.add_argument("--box", default="512,256,256")
-    ap.add_argument("--workload", default="ybif", choices=["ybif", "box"],
-                    help="ybif: the Y-bifurcation; box: an all-fluid box (kernel ceiling)")
+    ap.add_argument("--box", default=None, help="nx,ny,nz (default 512,256,256; 1024,512,512 for aneurysm)")
+    ap.add_argument("--workload", default="ybif", choices=["ybif", "box", "aneurysm"],
+                    help="ybif: the Y-bifurcation; box: an all-fluid box (kernel ceiling); "
+                         "aneurysm: the curved tube with a saccular bulge of the learned-start case")
     ap.add_argument("--wss-every", type=int, default=100)
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of oracle CPU work")
     ap.add_argument("--no-cpu", action="store_true")
@@ -109,15 +111,20 @@ def main():
 
     from pv_dec_original_amd import SubDomain, Simulation
     from pv_dec_original_amd import _native as N
-    from pv_dec_original_amd.workloads import ybif_case, fluid_box_case
+    from pv_dec_original_amd.workloads import aneurysm_case, fluid_box_case, inlet_disk_profile, ybif_case
 
     if N.device_count() < 1:
         raise SystemExit("no HIP device visible")
-    box = tuple(int(v) for v in args.box.split(","))
+    box = tuple(int(v) for v in (args.box or ("1024,512,512" if args.workload == "aneurysm" else "512,256,256")).split(","))
     q = args.lattice
     tau = 0.6
     t0 = time.perf_counter()
-    case = ybif_case(q=q, box=box, tau=tau) if args.workload == "ybif" else fluid_box_case(q, box, tau)
+    if args.workload == "ybif":
+        case = ybif_case(q=q, box=box, tau=tau)
+    elif args.workload == "aneurysm":
+        case = aneurysm_case(q=q, box=box, tau=tau)
+    else:
+        case = fluid_box_case(q, box, tau)
     t_geo = time.perf_counter() - t0
     log(f"geometry: {len(case.geometry.globalTriangles)} triangles, voxelised in {t_geo:.1f} s")
     t0 = time.perf_counter()
@@ -129,7 +136,10 @@ def main():
     nbox = box[0] * box[1] * box[2]
     log(f"setup {t_setup:.1f} s: {counts}")
     # parabolic inlet profile of unit mean (peak 2) on the parent tube
-    prof = case.inlet_profile(sd, 2.0)
+    if args.workload == "aneurysm":
+        prof = inlet_disk_profile(case.geometry, sd, 1.0)
+    else:
+        prof = case.inlet_profile(sd, 2.0)
     prof *= 1.0 / prof.mean()
     sim.set_inlet_profile(prof)
     sim.set_wss_every(args.wss_every)
