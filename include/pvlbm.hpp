@@ -12,6 +12,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -48,14 +49,30 @@ private:
  * constructor takes triangles already in memory and an optional explicit grid. */
 class Geometry {
 public:
+    /* ASCII or binary STL (GeometryReader.cpp:33-176): binary when the file is exactly the
+     * 84-byte header plus 50 bytes per facet its count declares. */
     Geometry(const std::string& FileName, const LatticeModel& velSet, double spacing, char raydirection = 'z')
         : velSet_(velSet), spacing(spacing), raydirection(raydirection) {
+        const bool binary = is_binary_stl(FileName);
+        auto read = binary ? pvlbm_read_stl_binary : pvlbm_read_stl_ascii;
         int64_t n = 0;
-        check(pvlbm_read_stl_ascii(FileName.c_str(), nullptr, 0, &n));
+        check(read(FileName.c_str(), nullptr, 0, &n));
         if (n == 0) throw Error(PVLBM_EIO, "no facets read from " + FileName);
         triangles.resize((size_t)(12 * n));
-        check(pvlbm_read_stl_ascii(FileName.c_str(), triangles.data(), n, &n));
+        check(read(FileName.c_str(), triangles.data(), n, &n));
         label(nullptr);
+    }
+    static bool is_binary_stl(const std::string& path) {
+        FILE* f = std::fopen(path.c_str(), "rb");
+        if (!f) return false;
+        unsigned char hdr[84];
+        const bool full = std::fread(hdr, 1, 84, f) == 84;
+        std::fseek(f, 0, SEEK_END);
+        const long size = std::ftell(f);
+        std::fclose(f);
+        if (!full) return false;
+        const uint32_t n = (uint32_t)hdr[80] | ((uint32_t)hdr[81] << 8) | ((uint32_t)hdr[82] << 16) | ((uint32_t)hdr[83] << 24);
+        return size == 84L + 50L * (long)n;
     }
     Geometry(std::vector<double> tri, const LatticeModel& velSet, double spacing,
              const pvlbm_grid* grid = nullptr)
@@ -121,6 +138,13 @@ public:
         check(pvlbm_get_kinds(h_, k.data()), h_);
         return k;
     }
+    /* Geometry::CalculateWallDistances (ComplexGeometry.cpp:550-608): q per wall node (list order)
+     * and direction, -1 without a link; ndirs = 19 is the reference's loop. */
+    std::vector<double> linkFractions(int ndirs = 19) const {
+        std::vector<double> q((size_t)(count(PVLBM_LIST_WALL) * geo_->velSet().Get_nDirections()));
+        check(pvlbm_link_fractions(h_, ndirs, q.data()), h_);
+        return q;
+    }
     std::vector<int64_t> InletBoundaryNode_() const { return list(PVLBM_LIST_INLET); }
     std::vector<int64_t> WallBoundaryNode_() const { return list(PVLBM_LIST_WALL); }
     std::vector<int64_t> nextToTheWall() const { return list(PVLBM_LIST_NTW); }
@@ -148,6 +172,23 @@ public:
     }
     void setInletProfile(const std::vector<double>& u) { check(pvlbm_set_inlet_profile(h_, u.data()), h_); }
     void setWssEvery(int every) { check(pvlbm_set_wss_every(h_, every), h_); }
+    // PVLBM_WALL_REFERENCE (the reference's wall rule) or PVLBM_WALL_INTERPOLATED
+    void setWallClosure(int mode) { check(pvlbm_set_wall_closure(h_, mode), h_); }
+    // restart from fields (rho [nz][ny][nx], u three such planes); empty vectors: from rest
+    void initFields(const std::vector<double>& rho, const std::vector<double>& u) {
+        check(pvlbm_init_fields(h_, rho.empty() ? nullptr : rho.data(), u.empty() ? nullptr : u.data()), h_);
+    }
+    int64_t iteration() const {
+        int64_t it = 0;
+        check(pvlbm_iteration(h_, &it), h_);
+        return it;
+    }
+    // FP64 (mass, inlet flux, outlet flux, non-finite count) of the last step
+    std::vector<double> reduce() const {
+        std::vector<double> r(4);
+        check(pvlbm_reduce(h_, r.data()), h_);
+        return r;
+    }
     void step(int n, const double* inlet_scale = nullptr) { check(pvlbm_step(h_, n, inlet_scale), h_); }
     static double getInletVelocity(double t, double period) { return pvlbm_pulsatile_velocity(t, period); }
 
@@ -187,6 +228,12 @@ public:
     std::vector<double> surfaceAttributes() const {
         std::vector<double> out((size_t)(6 * sd_.count(PVLBM_LIST_NTW)));
         if (!out.empty()) check(pvlbm_get_wss(h_, out.data()), h_);
+        return out;
+    }
+    // the WSS column of surfaceAttributes alone
+    std::vector<double> wssMagnitude() const {
+        std::vector<double> out((size_t)sd_.count(PVLBM_LIST_NTW));
+        if (!out.empty()) check(pvlbm_get_wss_magnitude(h_, out.data()), h_);
         return out;
     }
 

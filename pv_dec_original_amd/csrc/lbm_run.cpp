@@ -8,7 +8,7 @@
 //           [--storage fp64|fp16|bf16] [--tau 1.5] [--dt 8e-6]
 //           [--faces pv|x] [--face role,closure,axis,plane,tol,lo0,lo1,hi0,hi1,rho]...
 //           [--t-end 2.7] [--period 0.9] [--steps N --inlet const:V|sin:MEAN,AMP,PERIOD]
-//           [--output-every 100] [--outdir .] [--vtu]
+//           [--output-every 100] [--outdir .] [--vtu] [--wall reference|interpolated]
 //
 // --faces pv uses the reference's PV.stl planes (ComplexGeometry.cpp:1378-1408); --faces x puts the
 // inlet on the grid's x-min plane and a pressure outlet (rho 1) on its x-max plane.  Without
@@ -148,7 +148,7 @@ int usage() {
                  "               [--storage fp64|fp16|bf16] [--tau T] [--dt DT] [--faces pv|x]\n"
                  "               [--face role,closure,axis,plane,tol,lo0,lo1,hi0,hi1,rho]...\n"
                  "               [--t-end T] [--period P] [--steps N --inlet const:V|sin:M,A,P]\n"
-                 "               [--output-every K] [--outdir DIR] [--vtu]\n";
+                 "               [--output-every K] [--outdir DIR] [--vtu] [--wall reference|interpolated]\n";
     return 2;
 }
 
@@ -159,6 +159,7 @@ int main(int argc, char** argv) {
     double spacing = 0.05, tau = 1.5, dt = 8e-6, t_end = 2.7, period = 0.9;
     int lattice = 19, output_every = 100;
     bool vtu = false;
+    std::string wall = "reference";
     long steps = -1;
     std::vector<double> grid;
     std::vector<pvlbm_face> faces;
@@ -188,6 +189,7 @@ int main(int argc, char** argv) {
             else if (k == "--output-every") output_every = std::stoi(val());
             else if (k == "--outdir") outdir = val();
             else if (k == "--vtu") vtu = true;
+            else if (k == "--wall") wall = val();
             else if (k == "--help" || k == "-h") return usage(), 0;
             else return usage();
         } catch (const std::exception& e) {
@@ -227,6 +229,8 @@ int main(int argc, char** argv) {
         }
         pvlbm::SubDomain sd(*geo, faces, st, tau, dt);
         pvlbm::Simulation sim(sd, tau, 1.0, dt);
+        if (wall == "interpolated") sim.setWallClosure(PVLBM_WALL_INTERPOLATED);
+        else if (wall != "reference") throw std::runtime_error("--wall takes reference or interpolated");
         std::cerr << "grid " << geo->grid.dims[0] << "x" << geo->grid.dims[1] << "x" << geo->grid.dims[2]
                   << ", nodes " << sd.count(PVLBM_LIST_FLUID) << ", inlet " << sd.count(PVLBM_LIST_INLET)
                   << ", wall " << sd.count(PVLBM_LIST_WALL) << ", next-to-wall " << sd.count(PVLBM_LIST_NTW) << "\n";
