@@ -166,10 +166,13 @@ def test_warm_start_init_fields():
     assert np.abs(rho3[k] - 1.0).max() < 1e-14
 
 
-@pytest.mark.parametrize("storage,tol", [("fp16", 2e-2), ("bf16", 8e-2)])
-def test_shifted_storage_error_vs_fp64(storage, tol):
+# measured after 1500 steps: L2(u) 4.7e-4 - 6.3e-4 (FP16), 1.9e-3 - 3.7e-3 (BF16); WSS up to 8.9e-4 / 5.1e-3
+@pytest.mark.parametrize("q,case_name,storage,tol", [(19, "tube", "fp16", 2e-3), (19, "tube", "bf16", 1e-2),
+                                                      (27, "tube", "fp16", 2e-3), (27, "tube", "bf16", 1e-2),
+                                                      (27, "ybif", "fp16", 2e-3), (19, "ybif", "bf16", 1e-2)])
+def test_shifted_storage_error_vs_fp64(q, case_name, storage, tol):
     from pv_dec_original_amd.workloads import tube_case
-    case = tube_case(q=19)
+    case = tube_case(q=q) if case_name == "tube" else G.small_ybif_case(q=q)
     sd64, sim64, _ = G.build(case, "fp64")
     sd16, sim16, _ = G.build(case, storage)
     sim64.step(1500)
@@ -182,7 +185,9 @@ def test_shifted_storage_error_vs_fp64(storage, tol):
     assert e_u < tol, e_u
     assert e_r < 10 * tol
     w64, w16 = sim64.wss()[:, 3], sim16.wss()[:, 3]
-    assert np.linalg.norm(w16 - w64) / np.linalg.norm(w64) < 5 * tol
+    e_w = np.linalg.norm(w16 - w64) / np.linalg.norm(w64)
+    print(f"D3Q{q} {case_name} {storage}: L2(u) {e_u:.2e}, max|d rho| {e_r:.2e}, L2(WSS) {e_w:.2e}")
+    assert e_w < 5 * tol
 
 
 @pytest.mark.parametrize("which", ["tube19", "ybif27", "ybif27all"])
