@@ -188,42 +188,45 @@ __global__ __launch_bounds__(256) void k_tri_prep(const double* tri, int64_t ntr
 }
 
 // Normal of each listed node: sum over all triangles of unit normal * area / distance^2, then
-// normalised (Subdomain.cpp:343-368).  One block per node, FP64 block reduction.
+// normalised (Subdomain.cpp:343-368).  One thread per node, summing in triangle file order as the
+// reference does; the block's threads stage the per-triangle quantities in LDS, 256 triangles at a
+// time, so each is read from memory once per block instead of once per node.
 __global__ __launch_bounds__(256) void k_normals(const double* prep, int64_t ntri, const double* xyz,
                                                 int64_t n, double* nrm) {
-    const int64_t w = blockIdx.x;
-    const double x = xyz[w], y = xyz[n + w], z = xyz[2 * n + w];
+    constexpr int CH = 256;
+    __shared__ double P[7][CH];
+    const int64_t w = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const bool act = w < n;
+    const double x = act ? xyz[w] : 0.0, y = act ? xyz[n + w] : 0.0, z = act ? xyz[2 * n + w] : 0.0;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-    for (int64_t t = threadIdx.x; t < ntri; t += 256) {
-        double cX = prep[4 * ntri + t], cY = prep[5 * ntri + t], cZ = prep[6 * ntri + t];
-        double distance = sqrt((x - cX) * (x - cX) + (y - cY) * (y - cY) + (z - cZ) * (z - cZ));
-        double weight = prep[3 * ntri + t] / (distance * distance);
-        a0 += prep[t] * weight;
-        a1 += prep[ntri + t] * weight;
-        a2 += prep[2 * ntri + t] * weight;
-    }
-    __shared__ double s[3][256];
-    s[0][threadIdx.x] = a0;
-    s[1][threadIdx.x] = a1;
-    s[2][threadIdx.x] = a2;
-    __syncthreads();
-    for (int h = 128; h > 0; h >>= 1) {
-        if ((int)threadIdx.x < h)
-            for (int c = 0; c < 3; c++) s[c][threadIdx.x] += s[c][threadIdx.x + h];
+    for (int64_t t0 = 0; t0 < ntri; t0 += CH) {
+        const int64_t t = t0 + threadIdx.x;
+        __syncthreads();  // the previous chunk is consumed
+        if (t < ntri)
+            for (int c = 0; c < 7; c++) P[c][threadIdx.x] = prep[c * ntri + t];
         __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        double n0 = s[0][0], n1 = s[1][0], n2 = s[2][0];
-        double norm = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
-        if (norm > 0) {
-            n0 /= norm;
-            n1 /= norm;
-            n2 /= norm;
+        const int m = (int)(ntri - t0 < CH ? ntri - t0 : CH);
+#pragma unroll 4
+        for (int k = 0; k < m; k++) {
+            const double cX = P[4][k], cY = P[5][k], cZ = P[6][k];
+            const double distance = sqrt((x - cX) * (x - cX) + (y - cY) * (y - cY) + (z - cZ) * (z - cZ));
+            const double weight = P[3][k] / (distance * distance);
+            a0 += P[0][k] * weight;
+            a1 += P[1][k] * weight;
+            a2 += P[2][k] * weight;
         }
-        nrm[w] = n0;
-        nrm[n + w] = n1;
-        nrm[2 * n + w] = n2;
     }
+    if (!act) return;
+    double n0 = a0, n1 = a1, n2 = a2;
+    const double norm = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+    if (norm > 0) {
+        n0 /= norm;
+        n1 /= norm;
+        n2 /= norm;
+    }
+    nrm[w] = n0;
+    nrm[n + w] = n1;
+    nrm[2 * n + w] = n2;
 }
 
 }  // namespace
@@ -261,7 +264,7 @@ hipError_t launch_tri_prep(const double* tri, int64_t ntri, double* prep, hipStr
 hipError_t launch_normals(const double* prep, int64_t ntri, const double* xyz, int64_t n, double* nrm,
                           hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_normals, dim3((unsigned)n), dim3(256), 0, s, prep, ntri, xyz, n, nrm);
+    hipLaunchKernelGGL(k_normals, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, prep, ntri, xyz, n, nrm);
     return hipGetLastError();
 }
 

@@ -133,6 +133,7 @@ def test_wss_sampling_matches_oracle():
     orc.step(130, None, wss_every=25)
     w = sim.wss()
     wo = orc.wss()[0]
+    print(f"WSS max |GPU - oracle| {np.abs(w[:, 3] - wo[:, 3]).max():.3e}, TAWSS {np.abs(w[:, 5] - wo[:, 5]).max():.3e}")
     assert G.rel_err(w[:, 3], wo[:, 3]) <= TOL and G.rel_err(w[:, 5], wo[:, 5]) <= TOL
     assert np.array_equal(sim.wss_magnitude(), w[:, 3])
 
