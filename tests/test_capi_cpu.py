@@ -107,3 +107,13 @@ def test_csv_format(tmp_path):
     assert lines[1] == "1,-2.5,3.25,0.000123457,0.25,1.23457e+06"  # std::ostream defaults
     back = read_surface_csv(p)
     assert back.shape == (2, 6)
+
+
+def test_missing_library_fails_loudly(tmp_path, monkeypatch):
+    """Without the built library the engine raises instead of falling back to anything."""
+    from pv_dec_original_amd import _native
+    monkeypatch.setattr(_native, "HERE", str(tmp_path))
+    for name, loader in (("_lib", _native.lib), ("_net", _native.net), ("_gen", _native.gen)):
+        monkeypatch.setattr(_native, name, None)
+        with pytest.raises(_native.NativeError, match="missing"):
+            loader()
