@@ -4,8 +4,11 @@ storage started from the learned warm start (and the two ablations).
 Workload: a seeded curved tube of radius 40 with a spherical saccular bulge of radius 70 in a
 1024 x 512 x 512 box (workloads.aneurysm_case), D3Q19, tau = 0.6, parabolic inlet of unit mean
 scaled by u(t) = 0.03 + 0.02 sin(2 pi t / 4000), pressure outlet, WSS sampled every 200 steps.
-The network runs on the box scaled to 192 x 96 x 96 at equal Reynolds number; its field is
-upsampled trilinearly on the device (warmstart.WarmStart.apply, pvlbm_init_fields_coarse).
+The warm start: the network runs on the box scaled to 192 x 96 x 96 at equal Reynolds number (its
+input code gathered on the device), its steady field is upsampled trilinearly on the device, and
+the phase-0 plane fluxes and densities of the one-dimensional pulsatile model (phase0.py) are
+imposed on it (warmstart.WarmStart.apply, pvlbm_init_fields_coarse).  The network's weights are
+read and its buffers allocated once, before any run.
 
 Protocol: the period's WSS record is the WSS magnitude at every next-to-wall node, sampled every
 200 steps over one inlet period.  Each path stops when its period-to-period change falls below
