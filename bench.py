@@ -9,11 +9,14 @@ inlet periods.
 Output: one JSON line.  value = fluid-node updates per second / 1e6 (MLUPS) over the timed steps,
 inputs resident in HBM.  roofline.achieved = fluid nodes * 2 * Q * 8 bytes / average launch time.
 cpu_baseline = the CPU oracle (a restatement of the reference solver, push streaming) on a bounded
-number of steps of the same workload, single-threaded like one reference MPI rank.
+number of steps of the same workload, single-threaded like one reference MPI rank;
+cpu_baseline_threads = the same over the host cores the process may use (OpenMP).
+--workload box / aneurysm run the all-fluid box (the kernel's ceiling) or the aneurysm instead.
 
 "learned" (on by default, --learned 0 to skip): wall time to converged wall shear stress on the
 1024 x 512 x 512 aneurysm, the plain FP64 path from rest against FP16 shifted storage started from
-the learned warm start (pv_dec_original_amd/learned_bench.py).
+the learned warm start with the phase-0 profiles of the pulsatile model
+(pv_dec_original_amd/learned_bench.py).
 """
 import argparse
 import json
