@@ -19,6 +19,7 @@ the learned warm start with the phase-0 profiles of the pulsatile model
 (pv_dec_original_amd/learned_bench.py).
 """
 import argparse
+import ctypes as C
 import json
 import os
 import platform
@@ -129,7 +130,10 @@ def main():
     else:
         case = fluid_box_case(q, box, tau)
     t_geo = time.perf_counter() - t0
-    log(f"geometry: {len(case.geometry.globalTriangles)} triangles, voxelised in {t_geo:.1f} s")
+    vox_tests, vox_ms = C.c_int64(0), C.c_double(0.0)
+    N.lib().pvlbm_voxelize_stats(C.byref(vox_tests), C.byref(vox_ms))
+    log(f"geometry: {len(case.geometry.globalTriangles)} triangles, voxelised in {t_geo:.1f} s "
+        f"({vox_tests.value} ray-triangle tests in {vox_ms.value:.3f} ms of kernel time)")
     t0 = time.perf_counter()
     sd = SubDomain(case.geometry, case.faces, storage=args.storage, tau=tau, dt=8e-6)
     sim = Simulation(sd, tau, 1.0, 8e-6)
@@ -174,7 +178,6 @@ def main():
 
     copy_gbs = None
     if args.copy_bytes:
-        import ctypes as C
         cms = C.c_double(0)
         N.check(N.lib().pvlbm_copy_bench(args.copy_bytes, 10, C.byref(cms)))
         copy_gbs = 2 * (args.copy_bytes // 16 * 16) * 10 / (cms.value / 1e3) / 1e9
@@ -220,6 +223,9 @@ def main():
             "inlet": "0.03+0.02*sin(2*pi*t/4000) x parabolic(mean 1)", "wss_every": args.wss_every,
             "lists": counts, "wall_s_timed": round(wall, 3), "setup_s": round(t_setup, 2),
             "voxelise_s": round(t_geo, 2),
+            "voxelise_kernel_ms": round(vox_ms.value, 3), "ray_triangle_tests": vox_tests.value,
+            "ray_triangle_tests_per_s": (round(vox_tests.value / (vox_ms.value * 1e-3), -6)
+                                         if vox_ms.value > 0 else None),
         },
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -139,6 +139,12 @@ int pvlbm_grid_from_bbox(const double bbox[6], double h, pvlbm_grid* grid);
 int pvlbm_voxelize(const double* tri, int64_t ntri, double h, const pvlbm_grid* grid,
                    pvlbm_grid* grid_out, uint8_t* fluid_out);
 
+/* Work of this thread's last pvlbm_voxelize call that labelled points: the number of ray-triangle
+ * tests (every point of a column against the column's candidate triangles) and the labelling
+ * kernel's time in milliseconds (HIP events).  No counterpart in the reference (whose
+ * labelPoints tests every point against all triangles, ComplexGeometry.cpp:135-136, 1039). */
+int pvlbm_voxelize_stats(int64_t* tests, double* kernel_ms);
+
 /* Build a device-resident simulation from voxel flags: boundary detection over the Q
  * neighbours (ComplexGeometry.cpp:1259-1300), face classification (:1369-1429), node kinds
  * (Subdomain.cpp:91-169), neighbour links (:173-197), next-to-wall set (:201-218), area/d^2

@@ -81,6 +81,7 @@ PVLBM_API = [
     ("pvlbm_bbox", C.c_int, [_d, C.c_int64, _d]),
     ("pvlbm_grid_from_bbox", C.c_int, [_d, C.c_double, C.POINTER(Grid)]),
     ("pvlbm_voxelize", C.c_int, [_d, C.c_int64, C.c_double, C.POINTER(Grid), C.POINTER(Grid), _u8]),
+    ("pvlbm_voxelize_stats", C.c_int, [_i64, _d]),
     ("pvlbm_create", C.c_int, [_u8, C.POINTER(Grid), C.c_double, C.c_int, C.c_int, C.POINTER(Face),
                                C.c_int, _d, C.c_int64, C.c_double, C.c_double, C.POINTER(C.c_void_p)]),
     ("pvlbm_set_tau", C.c_int, [C.c_void_p, C.c_double]),

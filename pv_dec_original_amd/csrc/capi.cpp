@@ -367,6 +367,18 @@ extern "C" int pvlbm_grid_from_bbox(const double bbox[6], double h, pvlbm_grid* 
     });
 }
 
+namespace {
+// work and kernel time of this thread's last labelling call (pvlbm_voxelize_stats)
+thread_local int64_t g_vox_tests = 0;
+thread_local double g_vox_ms = 0.0;
+}  // namespace
+
+extern "C" int pvlbm_voxelize_stats(int64_t* tests, double* kernel_ms) {
+    if (tests) *tests = g_vox_tests;
+    if (kernel_ms) *kernel_ms = g_vox_ms;
+    return PVLBM_OK;
+}
+
 extern "C" int pvlbm_voxelize(const double* tri, int64_t ntri, double h, const pvlbm_grid* grid,
                               pvlbm_grid* grid_out, uint8_t* fluid_out) {
     return guarded(nullptr, [&] {
@@ -461,9 +473,22 @@ extern "C" int pvlbm_voxelize(const double* tri, int64_t ntri, double h, const p
                     p.ny = (int32_t)ny;
                     p.nz = (int32_t)nz;
                     p.fluid = d_fluid;
-                    HIPCHK(launch_voxelize(p, s));
-                    HIPCHK(hipMemcpyAsync(fluid_out, d_fluid, (size_t)(nx * ny * nz), hipMemcpyDeviceToHost, s));
-                    HIPCHK(hipStreamSynchronize(s));
+                    hipEvent_t e0 = nullptr, e1 = nullptr;
+                    HIPCHK(hipEventCreate(&e0));
+                    HIPCHK(hipEventCreate(&e1));
+                    HIPCHK(hipEventRecord(e0, s));
+                    hipError_t le = launch_voxelize(p, s);
+                    if (le == hipSuccess) le = hipEventRecord(e1, s);
+                    if (le == hipSuccess)
+                        le = hipMemcpyAsync(fluid_out, d_fluid, (size_t)(nx * ny * nz), hipMemcpyDeviceToHost, s);
+                    if (le == hipSuccess) le = hipStreamSynchronize(s);
+                    float ms = 0.f;
+                    if (le == hipSuccess) le = hipEventElapsedTime(&ms, e0, e1);
+                    (void)hipEventDestroy(e0);
+                    (void)hipEventDestroy(e1);
+                    HIPCHK(le);
+                    g_vox_tests = count[(size_t)(nx * ny)] * nz;
+                    g_vox_ms = (double)ms;
                 } catch (...) {
                     (void)hipStreamSynchronize(s);
                     dfree(d_tri); dfree(d_off); dfree(d_ids); dfree(d_fluid);
