@@ -35,8 +35,8 @@ def test_voxel_flags_bit_exact_tube():
     import ctypes as C
     tests, ms = C.c_int64(0), C.c_double(0.0)
     assert N.lib().pvlbm_voxelize_stats(C.byref(tests), C.byref(ms)) == 0
-    # every point meets at least the triangles of its column, at most all of them
-    assert 81 * 21 * 20 <= tests.value <= 81 * 21 * 20 * len(tri) and ms.value > 0
+    # each point meets its column's candidate triangles: some, and never more than all of them
+    assert 0 < tests.value <= 81 * 21 * 20 * len(tri) and ms.value > 0
     bb = O.bbox(tri)
     ref = O.label_points(tri, bb, dims, [bb[0], bb[2], bb[4]], 1.0)
     assert np.array_equal(fluid, ref)
