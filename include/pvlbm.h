@@ -263,6 +263,12 @@ int pvlbm_get_macros_device(pvlbm_handle h, float* rho, float* u);
  * evaluation (Simulation.cpp:880-905). */
 int pvlbm_get_wss(pvlbm_handle h, double* out6);
 
+/* Unit normals of the next-to-wall nodes, n3[3][nw] in next-to-wall list order: the
+ * area/distance^2-weighted sum over every triangle of SubDomain::calculateBoundaryNormals
+ * (Subdomain.cpp:224-370), as the WSS evaluation uses them; zero for a handle created without
+ * triangles. */
+int pvlbm_get_normals(pvlbm_handle h, double* n3);
+
 /* The WSS column of pvlbm_get_wss alone (nw doubles in next-to-wall list order: the magnitude of
  * Simulation.cpp:880-898 from the last WSS evaluation), without coordinates, OSI and TAWSS, for
  * callers that sample WSS often. */

@@ -597,6 +597,44 @@ void accumulate_wss(orc_sim* s, int64_t w, const double tau_i[3]) {
     s->acc[4 * w + 3] += WSS;
 }
 
+// calculateBoundaryNormals for one point (Subdomain.cpp:301-369): sum over every triangle, in
+// file order, of the unit face normal times area / distance^2 to the centroid, then normalised.
+void normal_at(const double* tri, int64_t ntri, double x, double y, double z, double out[3]) {
+    double n0 = 0.0, n1 = 0.0, n2 = 0.0;
+    for (int64_t t = 0; t < ntri; t++) {
+        const double* T = tri + 12 * t;
+        double ux = T[3] - T[0], uy = T[4] - T[1], uz = T[5] - T[2];
+        double vx = T[6] - T[0], vy = T[7] - T[1], vz = T[8] - T[2];
+        double nx = uy * vz - uz * vy;
+        double ny = uz * vx - ux * vz;
+        double nz = ux * vy - uy * vx;
+        double norm = std::sqrt(nx * nx + ny * ny + nz * nz);
+        if (norm > 0) {
+            nx /= norm;
+            ny /= norm;
+            nz /= norm;
+        }
+        double area = 0.5 * norm;
+        double cX = (T[0] + T[3] + T[6]) / 3.0;
+        double cY = (T[1] + T[4] + T[7]) / 3.0;
+        double cZ = (T[2] + T[5] + T[8]) / 3.0;
+        double distance = std::sqrt((x - cX) * (x - cX) + (y - cY) * (y - cY) + (z - cZ) * (z - cZ));
+        double weight = area / (distance * distance);
+        n0 += nx * weight;
+        n1 += ny * weight;
+        n2 += nz * weight;
+    }
+    double norm = std::sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+    if (norm > 0) {
+        n0 /= norm;
+        n1 /= norm;
+        n2 /= norm;
+    }
+    out[0] = n0;
+    out[1] = n1;
+    out[2] = n2;
+}
+
 }  // namespace
 
 extern "C" orc_sim* orc_sim_create(const uint8_t* fluid, const int32_t dims[3],
@@ -692,40 +730,8 @@ extern "C" orc_sim* orc_sim_create(const uint8_t* fluid, const int32_t dims[3],
         for (int64_t w = 0; w < nw; w++) {
             int64_t b = s->node_box[s->lists[PVLBM_LIST_NTW][w]];
             int64_t i = b % s->nx, j = (b / s->nx) % s->ny, k = b / (s->nx * s->ny);
-            double x = s->ox + (double)i * h, y = s->oy + (double)j * h, z = s->oz + (double)k * h;
-            double n0 = 0.0, n1 = 0.0, n2 = 0.0;
-            for (int64_t t = 0; t < ntri; t++) {
-                const double* T = tri + 12 * t;
-                double ux = T[3] - T[0], uy = T[4] - T[1], uz = T[5] - T[2];
-                double vx = T[6] - T[0], vy = T[7] - T[1], vz = T[8] - T[2];
-                double nx = uy * vz - uz * vy;
-                double ny = uz * vx - ux * vz;
-                double nz = ux * vy - uy * vx;
-                double norm = std::sqrt(nx * nx + ny * ny + nz * nz);
-                if (norm > 0) {
-                    nx /= norm;
-                    ny /= norm;
-                    nz /= norm;
-                }
-                double area = 0.5 * norm;
-                double cX = (T[0] + T[3] + T[6]) / 3.0;
-                double cY = (T[1] + T[4] + T[7]) / 3.0;
-                double cZ = (T[2] + T[5] + T[8]) / 3.0;
-                double distance = std::sqrt((x - cX) * (x - cX) + (y - cY) * (y - cY) + (z - cZ) * (z - cZ));
-                double weight = area / (distance * distance);
-                n0 += nx * weight;
-                n1 += ny * weight;
-                n2 += nz * weight;
-            }
-            double norm = std::sqrt(n0 * n0 + n1 * n1 + n2 * n2);
-            if (norm > 0) {
-                n0 /= norm;
-                n1 /= norm;
-                n2 /= norm;
-            }
-            s->normal[3 * w + 0] = n0;
-            s->normal[3 * w + 1] = n1;
-            s->normal[3 * w + 2] = n2;
+            normal_at(tri, ntri, s->ox + (double)i * h, s->oy + (double)j * h, s->oz + (double)k * h,
+                      &s->normal[3 * w]);
         }
     }
     // Initialize: f = w, f_new = 0 (Node.cpp:30-37), f_eq = 0 (Node.cpp:7), rho/u = 0
@@ -984,6 +990,21 @@ extern "C" void orc_sim_normals(const orc_sim* s, double* n3) {
     const int64_t nw = (int64_t)s->lists[PVLBM_LIST_NTW].size();
     for (int64_t w = 0; w < nw; w++)
         for (int c = 0; c < 3; c++) n3[c * nw + w] = s->normal[3 * w + c];
+}
+
+extern "C" void orc_sim_set_normals(orc_sim* s, const double* n3) {
+    const int64_t nw = (int64_t)s->lists[PVLBM_LIST_NTW].size();
+    for (int64_t w = 0; w < nw; w++)
+        for (int c = 0; c < 3; c++) s->normal[3 * w + c] = n3[c * nw + w];
+}
+
+extern "C" void orc_normal_at(const double* tri, int64_t ntri, const double* xyz, int64_t n, double* n3) {
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t p = 0; p < n; p++) {
+        double o[3];
+        normal_at(tri, ntri, xyz[3 * p], xyz[3 * p + 1], xyz[3 * p + 2], o);
+        for (int c = 0; c < 3; c++) n3[c * n + p] = o[c];
+    }
 }
 
 extern "C" int64_t orc_sim_gate_events(const orc_sim* s) { return s->gate; }

@@ -83,6 +83,10 @@ void orc_sim_set_populations(orc_sim* s, const double* f);
 void orc_sim_wss(const orc_sim* s, double* out6, double* tau3, double* acc4);
 /* normals of the next-to-wall nodes [3][nw] */
 void orc_sim_normals(const orc_sim* s, double* n3);
+// replaces the next-to-wall normals ([3][nw]); with normals = 0 at create they start at zero
+void orc_sim_set_normals(orc_sim* s, const double* n3);
+// reference normal (Subdomain.cpp:301-369) at n points xyz[n][3] -> n3[3][n]
+void orc_normal_at(const double* tri, int64_t ntri, const double* xyz, int64_t n, double* n3);
 int64_t orc_sim_gate_events(const orc_sim* s);
 
 /* Geometry::CalculateWallDistances (ComplexGeometry.cpp:550-608) with FindLineTriangleIntersection

@@ -104,6 +104,7 @@ PVLBM_API = [
     ("pvlbm_get_macros", C.c_int, [C.c_void_p, _d, _d]),
     ("pvlbm_get_macros_device", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     ("pvlbm_get_wss", C.c_int, [C.c_void_p, _d]),
+    ("pvlbm_get_normals", C.c_int, [C.c_void_p, _d]),
     ("pvlbm_get_wss_magnitude", C.c_int, [C.c_void_p, _d]),
     ("pvlbm_get_wss_raw", C.c_int, [C.c_void_p, _d, _d]),
     ("pvlbm_reduce", C.c_int, [C.c_void_p, _d]),

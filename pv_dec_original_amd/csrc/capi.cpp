@@ -1173,6 +1173,18 @@ extern "C" int pvlbm_get_macros_device(pvlbm_handle h, float* rho, float* u) {
     });
 }
 
+extern "C" int pvlbm_get_normals(pvlbm_handle h, double* n3) {
+    if (!h) return PVLBM_EINVAL;
+    return guarded(h, [&] {
+        if (!n3) fail(PVLBM_EINVAL, "n3 is NULL");
+        if (h->nw > 0) {
+            // device rows are [3][max(nw, 1)]; nw == max(nw, 1) here, so one contiguous copy
+            HIPCHK(hipMemcpyAsync(n3, h->d_nrm, sizeof(double) * 3 * h->nw, hipMemcpyDeviceToHost, h->stream));
+            HIPCHK(hipStreamSynchronize(h->stream));
+        }
+    });
+}
+
 extern "C" int pvlbm_get_wss_magnitude(pvlbm_handle h, double* out) {
     if (!h) return PVLBM_EINVAL;
     return guarded(h, [&] {

@@ -119,6 +119,13 @@ class SubDomain:
             self._chk(N.lib().pvlbm_link_fractions(self.handle, int(ndirs), N.dptr(q)))
         return q
 
+    def normals(self):
+        """[3][nw] unit normals of the next-to-wall nodes in list order (Subdomain.cpp:224-370)."""
+        n3 = np.zeros((3, self.count(N.LIST_NTW)))
+        if n3.shape[1]:
+            self._chk(N.lib().pvlbm_get_normals(self.handle, N.dptr(n3)))
+        return n3
+
     def kinds(self):
         k = np.zeros((self.nz, self.ny, self.nx), dtype=np.uint8)
         self._chk(N.lib().pvlbm_get_kinds(self.handle, N.u8ptr(k)))

@@ -56,6 +56,8 @@ def load():
         "orc_sim_set_populations": (None, [C.c_void_p, _d]),
         "orc_sim_wss": (None, [C.c_void_p, _d, _d, _d]),
         "orc_sim_normals": (None, [C.c_void_p, _d]),
+        "orc_sim_set_normals": (None, [C.c_void_p, _d]),
+        "orc_normal_at": (None, [_d, C.c_int64, _d, C.c_int64, _d]),
         "orc_sim_gate_events": (C.c_int64, [C.c_void_p]),
         "orc_collide": (None, [C.c_int, C.c_double, _d, _d, _d, _d]),
         "orc_zouhe": (None, [C.c_int, C.c_int, _d, C.c_double, C.c_double]),
@@ -115,6 +117,15 @@ def count_crossings(tri, start, end):
     tri = np.ascontiguousarray(tri, dtype=np.float64).reshape(-1, 12)
     return load().orc_count_crossings(_dp(tri), len(tri), _dp(np.asarray(start, dtype=np.float64)),
                                       _dp(np.asarray(end, dtype=np.float64)))
+
+
+def normal_at(tri, xyz):
+    """Reference normals [3][n] at the points xyz [n][3], every triangle in file order."""
+    tri = np.ascontiguousarray(tri, dtype=np.float64)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    out = np.zeros((3, len(xyz)))
+    load().orc_normal_at(_dp(tri), len(tri), _dp(xyz), len(xyz), _dp(out))
+    return out
 
 
 def lattice(q):
@@ -222,6 +233,12 @@ class OracleSim:
         if nw:
             self.L.orc_sim_normals(self.h, _dp(n3))
         return n3
+
+    def set_normals(self, n3):
+        nw = self.counts()["ntw"]
+        n3 = np.ascontiguousarray(n3, dtype=np.float64).reshape(3, nw)
+        if nw:
+            self.L.orc_sim_set_normals(self.h, _dp(n3))
 
     def link_fractions(self, tri, ndirs=19):
         """(box index per wall node, q [nwall][Q]) in the oracle's wall-list order."""

@@ -193,6 +193,10 @@ def test_normals_planar_and_radial(tube):
     # every side facet of this one-segment tube has its centroid at x = 40, so the area/d^2
     # weighting tilts normals away from radial with distance from x = 40
     assert np.median(cosang) > 0.99 and cosang.min() > 0.95
+    # the per-point helper is the same computation, and normals can be replaced
+    assert np.array_equal(O.normal_at(tube["tri"], xyz), n)
+    s.set_normals(-n)
+    assert np.array_equal(s.normals(), -n)
 
 
 @pytest.mark.parametrize("q", [19, 27])
