@@ -1,12 +1,15 @@
-"""The benchmark workload at its full size: the 512 x 256 x 256 Y-bifurcation (parent radius 24,
-daughters 17 at +-30 degrees, 850,648 triangles, 968,846 fluid nodes), D3Q27, FP64, against the
-oracle on the same inputs.
+"""The benchmark workloads at their full sizes against the oracle on the same inputs:
+
+- the 512 x 256 x 256 Y-bifurcation (parent radius 24, daughters 17 at +-30 degrees, 850,648
+  triangles), D3Q27 FP64: 968,846 nodes in 21,649 tiles of a 37.9 M-slot padded box;
+- the 1024 x 512 x 512 aneurysm box (curved tube radius 40 with a spherical bulge radius 70),
+  D3Q19 FP64: 6,453,194 nodes in a 285.8 M-slot padded box, whose populations span 2.3 GB per
+  plane of the kernel's 32-bit byte offsets.
 
 Voxel flags, node kinds and lists are compared whole.  The normals of 256 sampled next-to-wall
-nodes are checked against the reference sum over every triangle (all 115,714 would take the CPU a
-minute); the oracle then runs with the device's normals, so that the steps, WSS and the global sums
-are compared whole, bit for bit, at the size the benchmark measures (int64 and 32-bit offset
-arithmetic over a 37.9 M-slot padded box, 21,649 tiles, 5,413 reduction blocks).
+nodes are checked against the reference sum over every triangle (all of them would take the CPU
+minutes); the oracle then runs with the device's normals, so that the steps, WSS and the global
+sums are compared whole, bit for bit, at the sizes the benchmark measures.
 """
 import numpy as np
 import pytest
@@ -21,27 +24,34 @@ pytestmark = pytest.mark.gpu
 DT = 8e-6
 
 
-@pytest.fixture(scope="module")
-def full():
-    from pv_dec_original_amd.workloads import ybif_case
-    case = ybif_case(q=27)
+@pytest.fixture(scope="module", params=["ybif", "aneurysm"])
+def full(request):
+    from pv_dec_original_amd.workloads import aneurysm_case, inlet_disk_profile, ybif_case
+    if request.param == "ybif":
+        case = ybif_case(q=27)
+    else:
+        case = aneurysm_case(q=19)
     geo = case.geometry
     sd = SubDomain(geo, case.faces, storage="fp64", tau=case.tau, dt=DT)
     sim = Simulation(sd, case.tau, 1.0, DT)
-    prof = case.inlet_profile(sd, 2.0)
+    if request.param == "ybif":
+        prof = case.inlet_profile(sd, 2.0)
+    else:
+        prof = inlet_disk_profile(geo, sd, 1.0)
     prof *= 1.0 / prof.mean()
     sim.set_inlet_profile(prof)
-    orc = O.OracleSim(geo.fluid, geo.origin, geo.spacing, 27, case.faces, geo.globalTriangles, case.tau,
-                      DT, normals=False, threads=0)
+    orc = O.OracleSim(geo.fluid, geo.origin, geo.spacing, geo.m_velSet.q, case.faces, geo.globalTriangles,
+                      case.tau, DT, normals=False, threads=0)
     pbox = np.zeros(geo.fluid.size)
     pbox[sd.list(N.LIST_INLET)] = prof
     orc.set_inlet_profile_box(pbox)
-    print(f"full-size Y-bifurcation: {sd.counts()}", flush=True)
-    return case, sd, sim, orc
+    print(f"full-size {request.param}: {len(geo.globalTriangles)} triangles, {sd.counts()}", flush=True)
+    yield request.param, case, sd, sim, orc
+    del orc, sim, sd
 
 
 def test_full_size_voxel_flags_kinds_and_lists(full):
-    case, sd, sim, orc = full
+    name, case, sd, sim, orc = full
     geo = case.geometry
     tri = geo.globalTriangles
     box = (geo.nx, geo.ny, geo.nz)
@@ -51,14 +61,14 @@ def test_full_size_voxel_flags_kinds_and_lists(full):
         bb[2 * a + 1] = max(bb[2 * a + 1], geo.origin[a] + box[a] - 1)
     ref = O.label_points(tri, bb, box, geo.origin, 1.0, binned=True, threads=0)
     assert np.array_equal(geo.fluid, ref)
-    assert int(geo.fluid.sum()) == sd.count(N.LIST_FLUID) == 968846
+    assert int(geo.fluid.sum()) == sd.count(N.LIST_FLUID) == {"ybif": 968846, "aneurysm": 6453194}[name]
     assert np.array_equal(sd.kinds(), orc.kinds())
     for lid in (N.LIST_FLUID, N.LIST_INLET, N.LIST_OUTLET1, N.LIST_WALL, N.LIST_NTW):
         assert np.array_equal(np.sort(sd.list(lid)), np.sort(orc.list(lid))), lid
 
 
 def test_full_size_normals_sample(full):
-    case, sd, sim, orc = full
+    name, case, sd, sim, orc = full
     n_dev = sd.normals()
     nw = n_dev.shape[1]
     assert nw == sd.count(N.LIST_NTW) > 100000
@@ -71,7 +81,7 @@ def test_full_size_normals_sample(full):
 
 
 def test_full_size_fp64_steps_bit_exact(full):
-    case, sd, sim, orc = full
+    name, case, sd, sim, orc = full
     orc.set_normals(sd.normals())
     t = np.arange(60, dtype=np.float64)
     scale = 0.03 + 0.02 * np.sin(2.0 * np.pi * t / 4000.0)
@@ -80,11 +90,13 @@ def test_full_size_fp64_steps_bit_exact(full):
         orc.step(hi - lo, scale[lo:hi], wss_every=1)
         rho, u = sim.macros()
         rho_o, u_o = orc.macros()
-        d_rho, d_u = np.abs(rho - rho_o).max(), np.abs(u - u_o).max()
+        d_rho = np.abs(rho - rho_o).max()
+        d_u = max(np.abs(u[c] - u_o[c]).max() for c in range(3))
+        del rho_o, u_o
         w = sim.wss()
         wo = orc.wss()[0]
         d_w = np.abs(w[:, 3] - wo[:, 3]).max()
-        print(f"after {hi} steps: max |GPU - oracle| rho {d_rho:.1e} u {d_u:.1e} WSS {d_w:.1e}", flush=True)
+        print(f"{name} after {hi} steps: max |GPU - oracle| rho {d_rho:.1e} u {d_u:.1e} WSS {d_w:.1e}", flush=True)
         assert d_rho == 0 and d_u == 0 and d_w == 0
         assert np.array_equal(w[:, 5], wo[:, 5])
         tau_g, acc_g = sim.wss_raw()
