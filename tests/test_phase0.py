@@ -72,3 +72,35 @@ def test_phase0_profiles_shapes():
     rpl = np.linspace(1.02, 1.0, 40)
     t2, d2 = P.phase0_profiles(area, 3.0, 2.0, 500, 0.6, rpl)
     assert np.allclose(t2 - t, 3.0 * (rpl[0] / rpl - 1.0), atol=1e-12) and np.array_equal(d2, d)
+
+
+def test_convective_line_limits():
+    """The mean-flow option of the line: without mean flow it is the exact transfer above, and on
+    a uniform inviscid line (alpha ~ 5e4) with mean velocity U its solution is the pair of
+    Doppler-shifted plane waves p = P+ e^(-i w x / (c + U)) + P- e^(i w x / (c - U)) with
+    q = (A / c)(P+ e^(..) - P- e^(..)) (the acoustic impedance holds in the moving fluid), fixed by
+    q(0) = q_in and p(L) = 0."""
+    A = np.full(240, 800.0)
+    A[60:110] = 1500.0
+    q0, r0 = P.oscillation(A, 10.0, 2000.0, 0.6)
+    q1, r1 = P.oscillation(A, 10.0, 2000.0, 0.6, q_mean=1e-12)
+    assert np.allclose(q1, q0, rtol=0, atol=1e-9) and np.allclose(r1, r0, rtol=0, atol=1e-12)
+    n, a_, T, qa = 400, np.pi * 16.0 ** 2, 2000.0, 10.0
+    for U in (0.0, 0.03, 0.1):
+        qh, rh = P.oscillation(np.full(n, a_), qa, T, 0.5 + 1e-9, q_mean=U * a_ if U else 0.0)
+        c, w, L = np.sqrt(1.0 / 3.0), 2 * np.pi / T, n - 1
+        lp, lm = -1j * w / (c + U), 1j * w / (c - U)
+        qin = -1j * qa
+        # (A / c)(Pp - Pm) = qin,  Pp e^(lp L) + Pm e^(lm L) = 0
+        Pm = -(qin * c / a_) * np.exp(lp * L) / (np.exp(lp * L) + np.exp(lm * L))
+        Pp = qin * c / a_ + Pm
+        x = np.arange(n)
+        p_cf = Pp * np.exp(lp * x) + Pm * np.exp(lm * x)
+        q_cf = (a_ / c) * (Pp * np.exp(lp * x) - Pm * np.exp(lm * x))
+        assert np.abs(qh - q_cf).max() < 2e-3 * qa, U
+        assert np.abs(rh - 3.0 * p_cf).max() < 2e-3 * np.abs(3.0 * p_cf).max(), U
+    # the option reaches phase0_profiles
+    t0, d0 = P.phase0_profiles(A, 20.0, 10.0, 2000.0, 0.6)
+    t1, d1 = P.phase0_profiles(A, 20.0, 10.0, 2000.0, 0.6, convect=True)
+    qc, rc = P.oscillation(A, 10.0, 2000.0, 0.6, q_mean=20.0)
+    assert np.allclose(t1, 20.0 + qc.real) and np.allclose(d1, rc.real) and not np.allclose(t1, t0)
