@@ -1,6 +1,6 @@
 """Degenerate geometries on the device against the oracle: no fluid at all, fluid filling the
 whole grid (every face node pulls from padding or guard slots), and a slab one node thick (every
-node misses both links of the z axis)."""
+node misses both links of the z axis); and rejected calls on a live handle."""
 import numpy as np
 import pytest
 
@@ -63,3 +63,53 @@ def test_slab_one_node_thick(q):
     f[3, 2:10, :] = 1
     sd, u = _parity(_case(f, q), 120)
     assert sd.counts()["ntw"] == 0 and np.abs(u).max() > 1e-4
+
+
+def test_bad_arguments_are_errors_and_leave_the_state():
+    """On a live handle every rejected call returns its status (EINVAL = 1, ESTATE = 5) with a
+    message, launches nothing and leaves the state: 20 steps, a round of rejected calls, 20 more
+    steps equal 40 steps of a second handle, bit for bit."""
+    import ctypes as C
+    from pv_dec_original_amd.simulation import Simulation, SubDomain
+    EINVAL, ESTATE = 1, 5
+    case = G.small_ybif_case(q=19)
+    geo = case.geometry
+
+    def make():
+        sd = SubDomain(geo, case.faces, storage="fp64", tau=case.tau, dt=G.DT, triangles=False)
+        sim = Simulation(sd, case.tau, 1.0, G.DT)
+        sim.set_inlet_profile(case.inlet_profile(sd, 0.05))
+        return sd, sim
+
+    sd, sim = make()
+    L, h = N.lib(), sd.handle
+    msg = lambda: L.pvlbm_last_error(h).decode()
+    sim.step(20)
+    i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32).ctypes.data_as(C.POINTER(C.c_int32))
+    out = np.zeros(4, dtype=np.uint8)
+    ix, iy, iz = np.array([0, sd.nx]), np.array([1]), np.array([2, 3])
+    assert L.pvlbm_sample_kinds(h, i32(ix), 2, i32(iy), 1, i32(iz), 2, N.u8ptr(out)) == EINVAL
+    assert "outside" in msg()
+    q = np.zeros((max(sd.count(N.LIST_WALL), 1), 19))
+    assert L.pvlbm_link_fractions(h, 0, N.dptr(q)) == EINVAL and "ndirs" in msg()
+    assert L.pvlbm_link_fractions(h, 19, N.dptr(q)) == ESTATE and "triangles" in msg()
+    assert L.pvlbm_set_wall_closure(h, 7) == EINVAL
+    assert L.pvlbm_set_wall_closure(h, N.WALL_INTERPOLATED) == ESTATE and "triangles" in msg()
+    assert L.pvlbm_set_tau(h, 0.5) == EINVAL and L.pvlbm_set_wss_every(h, 0) == EINVAL
+    assert L.pvlbm_step(h, -1, None) == EINVAL
+    import torch
+    rc = torch.ones(8, dtype=torch.float32, device="cuda")  # valid device fields of a 2x2x2 grid
+    uc = torch.zeros(24, dtype=torch.float32, device="cuda")
+    nonfin = C.c_int64(0)
+    dp = lambda t: C.c_void_p(t.data_ptr())
+    assert L.pvlbm_init_fields_coarse(h, dp(rc), dp(uc), i32([2, 2, 0]), 0.1, 0.9, 1.1, 0.0, None, None,
+                                      C.byref(nonfin)) == EINVAL
+    assert L.pvlbm_init_fields_coarse(h, dp(rc), dp(uc), i32([2, 2, 2]), 0.0, 0.9, 1.1, 0.0, None, None,
+                                      C.byref(nonfin)) == EINVAL
+    torch.cuda.synchronize()
+    sim.step(20)
+    rho, u = sim.macros()
+    sd2, sim2 = make()
+    sim2.step(40)
+    rho2, u2 = sim2.macros()
+    assert np.array_equal(rho, rho2) and np.array_equal(u, u2) and sim.iteration == 40
