@@ -145,6 +145,13 @@ public:
         check(pvlbm_link_fractions(h_, ndirs, q.data()), h_);
         return q;
     }
+    /* SubDomain::calculateBoundaryNormals (Subdomain.cpp:224-370): unit normals [3][nw] of the
+     * next-to-wall nodes in list order. */
+    std::vector<double> normals() const {
+        std::vector<double> n3((size_t)(3 * count(PVLBM_LIST_NTW)));
+        if (!n3.empty()) check(pvlbm_get_normals(h_, n3.data()), h_);
+        return n3;
+    }
     std::vector<int64_t> InletBoundaryNode_() const { return list(PVLBM_LIST_INLET); }
     std::vector<int64_t> WallBoundaryNode_() const { return list(PVLBM_LIST_WALL); }
     std::vector<int64_t> nextToTheWall() const { return list(PVLBM_LIST_NTW); }
