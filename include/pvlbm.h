@@ -232,6 +232,17 @@ int pvlbm_link_fractions(pvlbm_handle h, int ndirs, double* q_out);
 enum { PVLBM_WALL_REFERENCE = 0, PVLBM_WALL_INTERPOLATED = 1 };
 int pvlbm_set_wall_closure(pvlbm_handle h, int mode);
 
+/* How the step writes the populations (no counterpart in the reference; results are identical
+ * under every policy).  PVLBM_STORES_AUTO (default): nontemporal stores when one step's
+ * algorithmic traffic, 2 * Q * bytes per population * nodes, exceeds PVLBM_STORES_AUTO_BYTES
+ * (128 MiB, half the MI355X's 256 MB last-level cache), plain stores below it, whose written
+ * lattice the next step then pulls from that cache.  PVLBM_STORES_REGULAR / _STREAMING force
+ * one.  pvlbm_get_store_policy reports the stores in use (1 = nontemporal). */
+enum { PVLBM_STORES_AUTO = 0, PVLBM_STORES_REGULAR = 1, PVLBM_STORES_STREAMING = 2 };
+#define PVLBM_STORES_AUTO_BYTES (128ll << 20)
+int pvlbm_set_store_policy(pvlbm_handle h, int policy);
+int pvlbm_get_store_policy(pvlbm_handle h, int* streaming);
+
 /* Every how many iterations WSS and its accumulators are evaluated (reference: every iteration,
  * Simulation.cpp:585-586).  Default 1. */
 int pvlbm_set_wss_every(pvlbm_handle h, int every);

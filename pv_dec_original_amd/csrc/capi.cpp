@@ -135,6 +135,8 @@ struct pvlbm_sim {
     std::vector<int64_t> bnode;
     std::vector<uint8_t> bclose;
     int wall_mode = PVLBM_WALL_REFERENCE;
+    int store_policy = PVLBM_STORES_AUTO;
+    int64_t nfluid = 0;
     float* d_bq = nullptr;
     // host-side lists (unpadded box indices) and slots
     std::vector<int64_t> lists[8];
@@ -144,6 +146,10 @@ struct pvlbm_sim {
     bool red_valid = false;
 
     size_t store_bytes() const { return storage == PVLBM_FP64 ? 8 : 2; }
+    bool stream_stores() const {
+        if (store_policy != PVLBM_STORES_AUTO) return store_policy == PVLBM_STORES_STREAMING;
+        return (int64_t)(2 * q * store_bytes()) * nfluid > PVLBM_STORES_AUTO_BYTES;
+    }
     size_t real_bytes() const { return storage == PVLBM_FP64 ? 8 : 4; }
     int blocks() const { return (ntiles + 3) / 4; }
     // padded row index of box row r = j + ny*k
@@ -176,6 +182,7 @@ struct pvlbm_sim {
         p.ntiles = ntiles;
         p.blocks = blocks();
         p.flags = flags;
+        p.stream_stores = stream_stores() ? 1 : 0;
         p.bmask = d_bmask;
         p.bclose = d_bclose;
         p.bprof = d_bprof;
@@ -658,6 +665,7 @@ extern "C" int pvlbm_create(const uint8_t* fluid, const pvlbm_grid* grid, double
             }
         }
         S->nw = (int64_t)wnode.size();
+        S->nfluid = (int64_t)S->lists[PVLBM_LIST_FLUID].size();
         S->ntiles = (int32_t)tiles.size();
         for (int32_t t : tiles) S->lists[PVLBM_LIST_TILES].push_back(t);
         // tile descriptors: lane masks and first side-table slots (slots follow node order)
@@ -1092,6 +1100,22 @@ extern "C" int pvlbm_set_wall_closure(pvlbm_handle h, int mode) {
             HIPCHK(hipStreamSynchronize(h->stream));
         }
         h->wall_mode = mode;
+    });
+}
+
+extern "C" int pvlbm_set_store_policy(pvlbm_handle h, int policy) {
+    if (!h) return PVLBM_EINVAL;
+    return guarded(h, [&] {
+        if (policy < PVLBM_STORES_AUTO || policy > PVLBM_STORES_STREAMING) fail(PVLBM_EINVAL, "unknown store policy %d", policy);
+        h->store_policy = policy;
+    });
+}
+
+extern "C" int pvlbm_get_store_policy(pvlbm_handle h, int* streaming) {
+    if (!h) return PVLBM_EINVAL;
+    return guarded(h, [&] {
+        if (!streaming) fail(PVLBM_EINVAL, "streaming is NULL");
+        *streaming = h->stream_stores() ? 1 : 0;
     });
 }
 

@@ -32,7 +32,6 @@ struct StFP64 {
     using S = double;
     using R = double;
     static constexpr bool kGate = true;
-    static constexpr bool kStreamStore = true;  // lattice larger than the caches: bypass them
     __device__ static R load(S v, double) { return v; }
     __device__ static S store(R v, double) { return v; }
 };
@@ -41,7 +40,6 @@ struct StFP16 {
     using S = _Float16;
     using R = float;
     static constexpr bool kGate = false;
-    static constexpr bool kStreamStore = false;
     __device__ static R load(S v, double w) { return (float)v + (float)w; }
     __device__ static S store(R v, double w) { return (_Float16)(v - (float)w); }
 };
@@ -50,7 +48,6 @@ struct StBF16 {
     using S = uint16_t;
     using R = float;
     static constexpr bool kGate = false;
-    static constexpr bool kStreamStore = false;
     __device__ static R load(S v, double w) { return __uint_as_float((uint32_t)v << 16) + (float)w; }
     __device__ static S store(R v, double w) {
         float x = v - (float)w;
@@ -137,7 +134,7 @@ __device__ __forceinline__ void wall_interpolated(const StepParams& p, uint32_t 
     }
 }
 
-template <int Q, class ST, int D = 0>
+template <int Q, class ST, bool NT, int D = 0>
 __device__ __forceinline__ void collide_store(const StepParams& p, uint32_t nb, typename ST::R rho,
                                               const typename ST::R (&u)[3],
                                               const typename ST::R (&f)[Q], typename ST::R omega) {
@@ -147,9 +144,9 @@ __device__ __forceinline__ void collide_store(const StepParams& p, uint32_t nb, 
         const R feq = feq_dir<Q, D>(rho, u);
         const R fo = f[D] + omega * (feq - f[D]);
         S* dst = reinterpret_cast<S*>(static_cast<char*>(p.B) + D * p.ns * (int64_t)sizeof(S) + nb);
-        if constexpr (ST::kStreamStore) __builtin_nontemporal_store(ST::store(fo, Lat<Q>::w[D]), dst);
+        if constexpr (NT) __builtin_nontemporal_store(ST::store(fo, Lat<Q>::w[D]), dst);
         else *dst = ST::store(fo, Lat<Q>::w[D]);
-        collide_store<Q, ST, D + 1>(p, nb, rho, u, f, omega);
+        collide_store<Q, ST, NT, D + 1>(p, nb, rho, u, f, omega);
     }
 }
 
@@ -165,7 +162,7 @@ __device__ __forceinline__ void init_populations(const StepParams& p, int64_t n,
     }
 }
 
-template <int Q, class ST, bool WSS>
+template <int Q, class ST, bool WSS, bool NT>
 __device__ __forceinline__ bool node_update(const StepParams& p, double (&red)[4]) {
     using R = typename ST::R;
     using S = typename ST::S;
@@ -273,7 +270,7 @@ __device__ __forceinline__ bool node_update(const StepParams& p, double (&red)[4
         M[3 * ms + ws] = u[2];
     }
 
-    collide_store<Q, ST>(p, nb, rho, u, f, R(p.omega));
+    collide_store<Q, ST, NT>(p, nb, rho, u, f, R(p.omega));
 
     if (p.flags & F_MACRO) {
         p.mrho[n] = (double)rho;
@@ -293,10 +290,10 @@ __device__ __forceinline__ bool node_update(const StepParams& p, double (&red)[4
     return true;
 }
 
-template <int Q, class ST, bool WSS>
+template <int Q, class ST, bool WSS, bool NT>
 __global__ __launch_bounds__(256) void k_step(const StepParams p) {
     double red[4] = {0.0, 0.0, 0.0, 0.0};
-    node_update<Q, ST, WSS>(p, red);
+    node_update<Q, ST, WSS, NT>(p, red);
     if (p.flags & F_REDUCE) {
         // FP64 block sums of (mass, inlet flux, outlet flux, non-finite count)
         __shared__ double s[4][4];
@@ -333,8 +330,15 @@ __global__ __launch_bounds__(256) void k_reduce_final(const double* partials, in
 template <int Q, class ST>
 void launch_q(const StepParams& p, hipStream_t s) {
     const dim3 g(p.blocks), b(256);
-    if (p.flags & F_WSS) hipLaunchKernelGGL((k_step<Q, ST, true>), g, b, 0, s, p);
-    else hipLaunchKernelGGL((k_step<Q, ST, false>), g, b, 0, s, p);
+    // nontemporal stores keep a lattice larger than the caches from evicting the pulls; a
+    // per-step working set that fits the last-level cache is served from it with plain stores
+    if (p.stream_stores) {
+        if (p.flags & F_WSS) hipLaunchKernelGGL((k_step<Q, ST, true, true>), g, b, 0, s, p);
+        else hipLaunchKernelGGL((k_step<Q, ST, false, true>), g, b, 0, s, p);
+    } else {
+        if (p.flags & F_WSS) hipLaunchKernelGGL((k_step<Q, ST, true, false>), g, b, 0, s, p);
+        else hipLaunchKernelGGL((k_step<Q, ST, false, false>), g, b, 0, s, p);
+    }
 }
 
 }  // namespace

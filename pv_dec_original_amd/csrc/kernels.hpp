@@ -41,7 +41,7 @@ struct StepParams {
     int32_t ntiles;
     int32_t blocks;         // launched blocks (4 tiles each)
     int32_t flags;
-    int32_t pad0;
+    int32_t stream_stores;  // 1: nontemporal population stores (per-step traffic above the cache)
     // boundary table (inlet, outlet and wall nodes)
     const uint32_t* bmask;  // bit d set: neighbour x + c_d is not a node
     const uint8_t* bclose;  // enum BClose

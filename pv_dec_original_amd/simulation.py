@@ -183,6 +183,17 @@ class Simulation:
         interpolated bounce-back on the wall links' surface fractions)."""
         self._chk(N.lib().pvlbm_set_wall_closure(self.handle, int(mode)))
 
+    def set_store_policy(self, policy):
+        """N.STORES_AUTO (default: nontemporal population stores when one step's traffic exceeds
+        128 MiB), N.STORES_REGULAR or N.STORES_STREAMING; results are the same under each."""
+        self._chk(N.lib().pvlbm_set_store_policy(self.handle, int(policy)))
+
+    def streaming_stores(self):
+        """True when the step writes the populations with nontemporal stores."""
+        v = C.c_int(0)
+        self._chk(N.lib().pvlbm_get_store_policy(self.handle, C.byref(v)))
+        return bool(v.value)
+
     def set_wss_every(self, every):
         self._chk(N.lib().pvlbm_set_wss_every(self.handle, int(every)))
 
