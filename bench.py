@@ -221,7 +221,8 @@ def main():
             "lattice": q, "storage": args.storage, "box": list(box), "fluid_nodes": nfl,
             "fluid_fraction": round(nfl / nbox, 4), "tau": tau,
             "inlet": "0.03+0.02*sin(2*pi*t/4000) x parabolic(mean 1)", "wss_every": args.wss_every,
-            "lists": counts, "wall_s_timed": round(wall, 3), "setup_s": round(t_setup, 2),
+            "lists": counts, "population_stores": "nontemporal" if sim.streaming_stores() else "plain",
+            "wall_s_timed": round(wall, 3), "setup_s": round(t_setup, 2),
             "voxelise_s": round(t_geo, 2),
             "voxelise_kernel_ms": round(vox_ms.value, 3), "ray_triangle_tests": vox_tests.value,
             "ray_triangle_tests_per_s": (round(vox_tests.value / (vox_ms.value * 1e-3), -6)

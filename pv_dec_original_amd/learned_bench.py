@@ -73,6 +73,8 @@ def run(weights=DEFAULT_WEIGHTS, box=(1024, 512, 512), tau=0.6, period=4000, sam
     out["fluid_nodes"] = int(sd64.counts()["fluid"])
     out["ntw_nodes"] = int(sd64.counts()["ntw"])
 
+    stores = {}
+
     def rest(sim):
         return lambda: (sim.init_fields(), "rest")[1]
 
@@ -80,6 +82,8 @@ def run(weights=DEFAULT_WEIGHTS, box=(1024, 512, 512), tau=0.6, period=4000, sam
         return lambda: W.start(sim, sd, warm, mean, tau, flux["q"], (amp, period) if phase0 else None)
 
     # the plain path, continued past its own convergence to a tighter reference record
+    for lab in ("fp64-rest", "fp64-warm"):
+        stores[lab] = "nontemporal" if sim64.streaming_stores() else "plain"
     plain = CV.run_periods(sim64, "fp64-rest", period, sample_every, mean, amp, max_periods,
                            stop_change=ref_change, start=rest(sim64), log=log)
     ref = plain.records[-1]
@@ -91,6 +95,8 @@ def run(weights=DEFAULT_WEIGHTS, box=(1024, 512, 512), tau=0.6, period=4000, sam
                                    stop_change=tol, reference=ref, start=learned(sim64, sd64), log=log))
     del sim64, sd64
     sd16, sim16 = make("fp16")
+    for lab in ("fp16-warm", "fp16-warm-mean", "fp16-rest"):
+        stores[lab] = "nontemporal" if sim16.streaming_stores() else "plain"
     runs.append(CV.run_periods(sim16, "fp16-warm", period, sample_every, mean, amp, max_periods,
                                stop_change=tol, reference=ref, start=learned(sim16, sd16), log=log))
     if ablations:
@@ -107,7 +113,7 @@ def run(weights=DEFAULT_WEIGHTS, box=(1024, 512, 512), tau=0.6, period=4000, sam
             "converged_period": own, "converged_wall_s": r.wall_at(own),
             "error_at_convergence": None if own is None else r.history[own - 1].get("error"),
             "within_tol_of_reference_period": to_ref, "within_tol_of_reference_wall_s": r.wall_at(to_ref),
-            "periods_run": r.periods, "kernel_ms": round(r.kernel_ms, 1),
+            "periods_run": r.periods, "kernel_ms": round(r.kernel_ms, 1), "population_stores": stores[r.label],
             "ms_per_step": round(r.kernel_ms / (r.periods * period), 4), "history": r.history}
     p, c = out["runs"]["fp64-rest"], out["runs"]["fp16-warm"]
 
