@@ -7,8 +7,10 @@ scaled by u(t) = 0.03 + 0.02 sin(2 pi t / 4000), pressure outlet, WSS sampled ev
 The warm start: the network runs on the box scaled to 192 x 96 x 96 at equal Reynolds number (its
 input code gathered on the device), its steady field is upsampled trilinearly on the device, and
 the phase-0 plane fluxes and densities of the one-dimensional pulsatile model (phase0.py) are
-imposed on it (warmstart.WarmStart.apply, pvlbm_init_fields_coarse).  The network's weights are
-read and its buffers allocated once, before any run.
+imposed on it (warmstart.WarmStart.apply, pvlbm_init_fields_coarse).  The model is loaded once,
+before any run, and its time reported as net_setup_s: the weights read, the buffers allocated,
+the phase-0 module imported and one inference on a synthetic tube (the first launch of every
+kernel on the start's path).
 
 Protocol: the period's WSS record is the WSS magnitude at every next-to-wall node, sampled every
 200 steps over one inlet period.  Each path stops when its period-to-period change falls below
@@ -68,7 +70,7 @@ def run(weights=DEFAULT_WEIGHTS, box=(1024, 512, 512), tau=0.6, period=4000, sam
            "reference": f"FP64 from rest until the period-to-period WSS change < {ref_change}",
            "criterion": f"each path stops when its period-to-period WSS change < {tol}; the chosen "
                         f"path's record then must be within {tol} (L2) of the reference"}
-    out["net_setup_s"] = round(net_setup_s, 3)  # weights read and buffers allocated, before any run
+    out["net_setup_s"] = round(net_setup_s, 3)  # model loaded (WarmStart + prepare), before any run
     sd64, sim64 = make("fp64")
     out["fluid_nodes"] = int(sd64.counts()["fluid"])
     out["ntw_nodes"] = int(sd64.counts()["ntw"])

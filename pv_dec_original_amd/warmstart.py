@@ -275,10 +275,21 @@ class WarmStart:
         return rho, u
 
     def prepare(self, shape):
-        """Allocate the network's buffers for a solver grid of shape (nz, ny, nx) ahead of the
-        first start (part of loading the model, like reading its weights)."""
-        cshape, _ = self.net_grid(shape)
+        """Load the model for a solver grid of shape (nz, ny, nx) ahead of the first start, like
+        reading its weights: the network's buffers, the phase-0 model's module, and one
+        inference with its rescaling on a straight tube of the network grid, so that the
+        first start does not pay the first launch of those kernels."""
+        cshape, s = self.net_grid(shape)
         self.net.buffers((1,) + tuple(cshape))
+        from . import phase0  # noqa: F401  (scipy.special)
+        cz, cy, cx = cshape
+        z, y = np.meshgrid(np.arange(cz), np.arange(cy), indexing="ij")
+        disk = (z - 0.5 * cz) ** 2 + (y - 0.5 * cy) ** 2 < (0.25 * min(cz, cy)) ** 2
+        cc = np.zeros(cshape, dtype=np.uint8)
+        cc[disk, :] = 1
+        cc[disk, 0] = 2
+        cc[disk, -1] = 3
+        self._fields_from_code(cc, 0.03, 0.7, s)
 
     def coarse_fields(self, kinds, u_mean, tau, sd=None):
         """The network's field for a solver grid, on the network grid: device tensors rho
@@ -287,6 +298,10 @@ class WarmStart:
         then gathered on the device, coarse_code_subdomain)."""
         cshape, s = self.net_grid(kinds.shape if kinds is not None else (sd.nz, sd.ny, sd.nx))
         cc = coarse_code(kinds, cshape) if kinds is not None else coarse_code_subdomain(sd, cshape)
+        rho, u = self._fields_from_code(cc, u_mean, tau, s)
+        return rho, u, cc
+
+    def _fields_from_code(self, cc, u_mean, tau, s):
         uc, tc = coarse_params(u_mean, tau, s) if s > 1 else (u_mean, tau)
         rho, u = self.predict(cc, uc, tc)
         # at equal Reynolds number u / u_mean is the same field, and rho - 1 (viscous drop
@@ -296,7 +311,7 @@ class WarmStart:
         solid = torch.from_numpy(cc == 0).to("cuda")
         rho = torch.where(solid, torch.ones_like(rho), 1.0 + (rho - 1.0) * (k * k))
         u = torch.where(solid[None], torch.zeros_like(u), u * k)
-        return rho, u, cc
+        return rho, u
 
     def apply(self, sim, sd, u_mean, tau, flux=None, pulsatile=None):
         """Start sim from the network's field (True) or leave it untouched when the field fails
