@@ -28,14 +28,17 @@ typedef struct {
     int32_t nb, d, h, w; /* batch of nb boxes of d x h x w voxels (z, y, x; x fastest) */
 } pvnet_shape;
 
-enum { PVNET_RELU = 1, PVNET_F32OUT = 2 };
+enum { PVNET_RELU = 1, PVNET_F32OUT = 2, PVNET_GATHER = 4 };
 
 /* y[m][n] = act(bias[n] + sum_{tap, ci} x[m + off(tap)][ci] * w[n][tap*cin + ci])
  * ntaps = 27: 3x3x3 stencil with zero padding, tap = (dz+1)*9 + (dy+1)*3 + (dx+1);
  * ntaps = 1: pointwise.  w: bf16 [cout][kpad], kpad = roundup(ntaps*cin, 32), zero beyond
  * ntaps*cin.  If relu_of is not NULL the result is multiplied by (relu_of[m][n] > 0) (the
  * backward of a ReLU whose output is relu_of).  flags: PVNET_RELU applies max(., 0);
- * PVNET_F32OUT writes float instead of bf16.  y may be a channel slice of a wider tensor. */
+ * PVNET_F32OUT writes float instead of bf16.  y may be a channel slice of a wider tensor.
+ * 16-output 3x3x3 convolutions on grids of whole 8 x 8 x 4 bricks stage each brick and its halo
+ * in LDS once; PVNET_GATHER keeps them on the per-tap gathering kernel (the same result bit for
+ * bit, for comparison). */
 typedef struct {
     pvnet_shape s;
     int32_t ntaps, cin, cout, kpad;

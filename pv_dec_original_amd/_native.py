@@ -67,7 +67,7 @@ class NetWgrad(C.Structure):
                 ("ws", C.c_void_p), ("dw", C.c_void_p), ("db", C.c_void_p)]
 
 
-NET_RELU, NET_F32OUT = 1, 2
+NET_RELU, NET_F32OUT, NET_GATHER = 1, 2, 4
 
 _lib = None
 _gen = None

@@ -171,6 +171,86 @@ __global__ __launch_bounds__(256) void k_conv_fwd(pvnet_conv a, int M, int K) {
 }
 
 // ---------------------------------------------------------------------------------------------
+// 3x3x3 convolution with 16 output channels on grids of whole 8 x 8 x 4 bricks: each block stages
+// one brick of one box with its one-voxel halo (10 x 10 x 6 voxels, all cin channels) in LDS once
+// and runs the k-steps of k_conv_fwd over it, where k_conv_fwd gathers every tap's rows from L2
+// again.  The k-steps, their operands and their order are those of k_conv_fwd, so each output is
+// the same bit for bit.  Wave wv computes brick plane z = wv as 4 MFMA row tiles of 16 voxels
+// (x = c % 8, y = 2 tile + c / 8).
+
+constexpr int BRX = 8, BRY = 8, BRZ = 4;
+constexpr int HLX = BRX + 2, HLY = BRY + 2, HLZ = BRZ + 2, HLV = HLX * HLY * HLZ;
+
+__global__ __launch_bounds__(256) void k_conv_fwd_brick(pvnet_conv a, int K) {
+    extern __shared__ uint4 halo[];  // [HLV voxels][cin / 8 chunks of 8 bf16]
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int g = lane >> 4, c16 = lane & 15;
+    const int d = a.s.d, h = a.s.h, w = a.s.w;
+    const int nbx = w / BRX, nby = h / BRY, nbz = d / BRZ;
+    int bid = blockIdx.x;
+    const int bx = bid % nbx;
+    bid /= nbx;
+    const int by = bid % nby;
+    bid /= nby;
+    const int bz = bid % nbz;
+    const int box = bid / nbz;
+    const int x0 = bx * BRX, y0 = by * BRY, z0 = bz * BRZ;
+    const int cch = a.cin >> 3;
+    for (int i = t; i < HLV * cch; i += 256) {
+        const int v = i / cch, ch = i - v * cch;
+        const int x = x0 - 1 + v % HLX, y = y0 - 1 + (v / HLX) % HLY, z = z0 - 1 + v / (HLX * HLY);
+        uint4 val = make_uint4(0, 0, 0, 0);
+        if ((unsigned)x < (unsigned)w && (unsigned)y < (unsigned)h && (unsigned)z < (unsigned)d) {
+            const int64_t m = (((int64_t)box * d + z) * h + y) * w + x;
+            val = *(const uint4*)(a.x + m * a.ldx + ch * 8);
+        }
+        halo[i] = val;
+    }
+    __syncthreads();
+
+    f32x4 acc[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int xl = c16 & 7, yl = c16 >> 3;
+    const int nsteps = a.kpad / BK;
+    for (int s = 0; s < nsteps; s++) {
+        const int k = s * BK + g * 8;
+        const bool kin = k < K;
+        const int tap = kin ? k / a.cin : 0;
+        const int ci = k - tap * a.cin;
+        int dz, dy, dx;
+        tap_offset(27, tap, dz, dy, dx);
+        const bf16x8 bfr = *(const bf16x8*)(a.w + (int64_t)c16 * a.kpad + s * BK + g * 8);
+        // halo voxel of (xl, 2 i + yl, wv) shifted by the tap
+        const int base = ((wv + 1 + dz) * HLY + (yl + 1 + dy)) * HLX + (xl + 1 + dx);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            uint4 r = make_uint4(0, 0, 0, 0);
+            if (kin) r = halo[(base + 2 * i * HLX) * cch + (ci >> 3)];
+            acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(*reinterpret_cast<const bf16x8*>(&r), bfr, acc[i], 0, 0, 0);
+        }
+    }
+
+    const bool relu = a.flags & PVNET_RELU, f32out = a.flags & PVNET_F32OUT;
+    const int n = c16;
+    const float bias = a.bias ? a.bias[n] : 0.f;
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int r = 4 * g + e;  // MFMA row: voxel (r % 8, 2 i + r / 8) of plane wv
+            const int64_t m = (((int64_t)box * d + z0 + wv) * h + y0 + 2 * i + (r >> 3)) * w + x0 + (r & 7);
+            float v = acc[i][e] + bias;
+            if (relu) v = v < 0.f ? 0.f : v;
+            if (a.relu_of && !bf_pos(a.relu_of[m * a.ldr + n])) v = 0.f;
+            if (f32out)
+                ((float*)a.y)[m * a.ldy + n] = v;
+            else
+                ((uint16_t*)a.y)[m * a.ldy + n] = f2bf(v);
+        }
+}
+
+// ---------------------------------------------------------------------------------------------
 // weight gradient: block tile BN output channels x 64 k-values, reduction over its voxel chunk in
 // LDS fills of 128 voxels.  Threads 0..127 each load an 8-voxel x 8-channel block of the gathered
 // input (eight 16-byte loads), transpose it in registers and write eight 16-byte LDS rows of 8
@@ -669,7 +749,11 @@ int pvnet_conv_fwd(const pvnet_conv* c, void* stream) {
     // tiles: 256-voxel blocks for narrow outputs on large grids; 64-channel tiles only when the
     // grid still has >= 2048 blocks (the K loop is long, so the device needs many blocks in flight)
     const int mt = blocks_for(M, 128);
-    if (c->cout <= 16 && M >= (int64_t)256 * 2048) {
+    const size_t halo_bytes = (size_t)HLV * c->cin * 2;
+    if (c->cout == 16 && c->ntaps == 27 && !(c->flags & PVNET_GATHER) && c->s.d % BRZ == 0 &&
+        c->s.h % BRY == 0 && c->s.w % BRX == 0 && halo_bytes <= 64 * 1024 && M >= (int64_t)256 * 2048) {
+        hipLaunchKernelGGL(k_conv_fwd_brick, dim3((unsigned)(M / (BRX * BRY * BRZ))), dim3(256), halo_bytes, st, *c, K);
+    } else if (c->cout <= 16 && M >= (int64_t)256 * 2048) {
         hipLaunchKernelGGL((k_conv_fwd<16, 64>), dim3(blocks_for(M, 256), 1), dim3(256), 0, st, *c, (int)M, K);
     } else if (c->cout <= 16) {
         hipLaunchKernelGGL((k_conv_fwd<16, 32>), dim3(mt, 1), dim3(256), 0, st, *c, (int)M, K);

@@ -165,6 +165,43 @@ def test_conv_fwd_large_grid_tiles(T):
         _close_bf16(got[m], ref, f"voxel {m}")
 
 
+@pytest.mark.parametrize("cin,flags,ldx,xoff,ldy,yoff,mask", [
+    (8, 1, 8, 0, 16, 0, False), (16, 1, 24, 8, 48, 32, False), (32, 2, 32, 0, 16, 0, False),
+    (48, 0, 48, 0, 16, 0, True)])
+def test_conv_fwd_brick_equals_gather(T, cin, flags, ldx, xoff, ldy, yoff, mask):
+    """16-output 3x3x3 convolutions on whole 8 x 8 x 4 bricks stage each brick with its halo in
+    LDS; the result must equal the per-tap gathering kernel's (PVNET_GATHER) bit for bit, over two
+    boxes (zero padding at every box face), channel slices of wider tensors, ReLU, FP32 output and
+    the ReLU-backward mask.  Columns outside the output slice stay untouched."""
+    from pv_dec_original_amd import _native as N
+    NL = N.net()
+    shape, cout = (2, 32, 96, 96), 16
+    M = O.nvox(shape)
+    xfull = np.zeros((M, ldx))
+    xfull[:, xoff:xoff + cin] = _rbf((M, cin))
+    w = O.bf16_round(_rbf((cout, 27 * cin)) / np.sqrt(27 * cin))
+    b = RNG.standard_normal(cout).astype(np.float32).astype(np.float64)
+    xd, w32, bd = _dev(T, xfull), _dev(T, w, T.float32), _dev(T, b, T.float32)
+    kpad = (27 * cin + 31) // 32 * 32
+    wt = T.zeros((cout, kpad), dtype=T.bfloat16, device="cuda")
+    wdp = T.zeros((cin, (27 * cout + 31) // 32 * 32), dtype=T.bfloat16, device="cuda")
+    N.net_check(NL.pvnet_pack_weights(_p(w32), 27, cin, cout, _p(wt), _p(wdp), _st(T)))
+    rel = _dev(T, _rbf((M, cout))) if mask else None
+    f32 = bool(flags & 2)
+    out = {}
+    for gather in (False, True):
+        yd = T.full((M, ldy), 7.0, dtype=T.float32 if f32 else T.bfloat16, device="cuda")
+        c = N.NetConv(N.NetShape(*shape), 27, cin, cout, kpad, _p(xd, xoff), ldx, flags | (N.NET_GATHER if gather else 0),
+                      _p(wt), _p(bd), _p(yd, yoff), ldy, cout if mask else 0, _p(rel) if mask else None)
+        N.net_check(NL.pvnet_conv_fwd(C.byref(c), _st(T)))
+        T.cuda.synchronize()
+        out[gather] = yd
+    a, g = out[False], out[True]
+    assert T.equal(a, g), f"{(a != g).sum().item()} of {a.numel()} differ"
+    assert (a[:, :yoff] == 7.0).all() and (a[:, yoff + cout:] == 7.0).all()
+    assert a[:, yoff:yoff + cout].abs().max().item() > 0
+
+
 def test_wgrad_deterministic(T):
     from pv_dec_original_amd import _native as N
     NL = N.net()
