@@ -104,3 +104,17 @@ def test_convective_line_limits():
     t1, d1 = P.phase0_profiles(A, 20.0, 10.0, 2000.0, 0.6, convect=True)
     qc, rc = P.oscillation(A, 10.0, 2000.0, 0.6, q_mean=20.0)
     assert np.allclose(t1, 20.0 + qc.real) and np.allclose(d1, rc.real) and not np.allclose(t1, t0)
+
+
+def test_matrix_exponential_closed_form():
+    """_expm2 (e^t (cosh s I + sinh(s)/s N), N = M - t I) against scipy's Pade exponential on
+    random complex 2x2 matrices, including nearly nilpotent N (the sinh(s)/s series branch)."""
+    from scipy.linalg import expm
+    rng = np.random.default_rng(4)
+    M = rng.standard_normal((40, 2, 2)) + 1j * rng.standard_normal((40, 2, 2))
+    M[:10] *= 1e-3
+    N = np.array([[0.0, 1.0], [0.0, 0.0]], dtype=complex)  # nilpotent: exp = I + N
+    M[10] = 0.3 * np.eye(2) + 1e-9 * N
+    got = P._expm2(M)
+    ref = np.array([expm(m) for m in M])
+    assert np.allclose(got, ref, rtol=1e-12, atol=1e-14)
