@@ -178,6 +178,14 @@ struct pvlbm_sim {
         p.guard = guard;
         p.sy = nxp;
         p.sz = nxp * nyp;
+        {
+            const int64_t S = (int64_t)store_bytes();
+            const int* cx = (q == 19) ? Lat<19>::cx : Lat<27>::cx;
+            const int* cy = (q == 19) ? Lat<19>::cy : Lat<27>::cy;
+            const int* cz = (q == 19) ? Lat<19>::cz : Lat<27>::cz;
+            p.poff_plane = ns * S;
+            for (int d = 0; d < q; d++) p.poff_pull[d] = d * p.poff_plane - (cx[d] + cy[d] * p.sy + cz[d] * p.sz) * S;
+        }
         p.tdesc = d_tdesc;
         p.ntiles = ntiles;
         p.blocks = blocks();

@@ -15,8 +15,9 @@
 // node, next-to-wall, boundary) and the first side-table slot of each kind, so a lane finds its
 // slot with mbcnt and no per-lane lookup.  Each pull is one contiguous 512-byte (FP64) / 128-byte
 // (16-bit) run of a direction plane shifted by the stencil offset; every plane has a guard before
-// and after so that all pull addresses are in bounds and the per-lane part of an address is an
-// unsigned 32-bit byte offset from a scalar plane base.
+// and after so that all pull addresses are in bounds.  An address is a 64-bit wave-uniform base
+// (plane, stencil offset and tile) plus the lane's byte offset within the tile, so one VGPR serves
+// every direction and a plane may exceed 4 GiB.
 
 #include <hip/hip_runtime.h>
 
@@ -70,20 +71,20 @@ __device__ __forceinline__ int32_t lanes_below(uint64_t m) {
     return (int32_t)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
-template <int Q, class S, int D>
-__device__ __forceinline__ const S* pull_addr(const StepParams& p, uint32_t nb) {
-    constexpr int64_t cx = Lat<Q>::cx[D], cy = Lat<Q>::cy[D], cz = Lat<Q>::cz[D];
-    const int64_t offb = (cx + cy * p.sy + cz * p.sz) * (int64_t)sizeof(S);
-    const char* plane = static_cast<const char*>(p.A) + D * p.ns * (int64_t)sizeof(S);
-    return reinterpret_cast<const S*>(plane + (uint32_t)(nb - (uint32_t)offb));
+// Addresses: `tb` is the byte index of the tile's first lane within a plane (wave-uniform, 64-bit)
+// and `lo` the lane's byte offset within the tile (lane * sizeof(S)), so that each access is a
+// 64-bit scalar base per direction plus the one 32-bit lane offset all directions share.
+template <class S, class P>
+__device__ __forceinline__ S* at(P* base, int64_t sbytes, uint32_t lo) {
+    return reinterpret_cast<S*>(reinterpret_cast<typename std::conditional<std::is_const<P>::value, const char, char>::type*>(base) + sbytes + lo);
 }
 
-// Interior lanes: all Q pulls issued back to back from scalar plane bases.
+// Interior lanes: all Q pulls issued back to back, direction D from slot x - c_D of plane D.
 template <int Q, class ST, int D = 0>
-__device__ __forceinline__ void load_all(const StepParams& p, uint32_t nb, typename ST::S (&v)[Q]) {
+__device__ __forceinline__ void load_all(const StepParams& p, int64_t tb, uint32_t lo, typename ST::S (&v)[Q]) {
     if constexpr (D < Q) {
-        v[D] = *pull_addr<Q, typename ST::S, D>(p, nb);
-        load_all<Q, ST, D + 1>(p, nb, v);
+        v[D] = *at<const typename ST::S>(p.A, p.poff_pull[D] + tb, lo);
+        load_all<Q, ST, D + 1>(p, tb, lo, v);
     }
 }
 
@@ -91,17 +92,12 @@ __device__ __forceinline__ void load_all(const StepParams& p, uint32_t nb, typen
 // slot x - c_D of plane D, a non-node slot (solid, or padding / guard outside the box) that only x
 // ever addresses in that plane, so x parks its post-closure value there for the next step.
 template <int Q, class ST, int D = 0>
-__device__ __forceinline__ void save_stale(const StepParams& p, uint32_t nb, uint32_t nmiss,
+__device__ __forceinline__ void save_stale(const StepParams& p, int64_t tb, uint32_t lo, uint32_t nmiss,
                                            const typename ST::R (&f)[Q]) {
     if constexpr (D < Q) {
         using S = typename ST::S;
-        if ((nmiss >> Lat<Q>::opp[D]) & 1u) {
-            constexpr int64_t cx = Lat<Q>::cx[D], cy = Lat<Q>::cy[D], cz = Lat<Q>::cz[D];
-            const int64_t offb = (cx + cy * p.sy + cz * p.sz) * (int64_t)sizeof(S);
-            char* plane = static_cast<char*>(p.B) + D * p.ns * (int64_t)sizeof(S);
-            *reinterpret_cast<S*>(plane + (uint32_t)(nb - (uint32_t)offb)) = ST::store(f[D], Lat<Q>::w[D]);
-        }
-        save_stale<Q, ST, D + 1>(p, nb, nmiss, f);
+        if ((nmiss >> Lat<Q>::opp[D]) & 1u) *at<S>(p.B, p.poff_pull[D] + tb, lo) = ST::store(f[D], Lat<Q>::w[D]);
+        save_stale<Q, ST, D + 1>(p, tb, lo, nmiss, f);
     }
 }
 
@@ -111,31 +107,30 @@ __device__ __forceinline__ void save_stale(const StepParams& p, uint32_t nb, uin
 // the previous launch, read from its own slot of A; f*_I(x - c_I) is the population it pulled in
 // direction I (valid when x - c_I, i.e. x + c_D, is a node).
 template <int Q, class ST, int D = 0>
-__device__ __forceinline__ void wall_interpolated(const StepParams& p, uint32_t nb, uint32_t nmiss,
+__device__ __forceinline__ void wall_interpolated(const StepParams& p, int64_t tb, uint32_t lo, uint32_t nmiss,
                                                   const float* bq, typename ST::R (&f)[Q]) {
     if constexpr (D < Q) {
         using R = typename ST::R;
         using S = typename ST::S;
         constexpr int I = Lat<Q>::opp[D];
         if ((nmiss >> I) & 1u) {
-            const char* base = static_cast<const char*>(p.A) + nb;
-            const R aI = ST::load(*reinterpret_cast<const S*>(base + I * p.ns * (int64_t)sizeof(S)), Lat<Q>::w[I]);
+            const R aI = ST::load(*at<const S>(p.A, I * p.poff_plane + tb, lo), Lat<Q>::w[I]);
             const R q = R(bq[I]);
             R v = aI;  // no crossing found, or a thin wall: halfway bounce-back
             if (q >= R(0.5)) {
-                const R aD = ST::load(*reinterpret_cast<const S*>(base + D * p.ns * (int64_t)sizeof(S)), Lat<Q>::w[D]);
+                const R aD = ST::load(*at<const S>(p.A, D * p.poff_plane + tb, lo), Lat<Q>::w[D]);
                 v = aI / (R(2) * q) + (R(2) * q - R(1)) / (R(2) * q) * aD;
             } else if (q >= R(0) && !((nmiss >> D) & 1u)) {
                 v = R(2) * q * aI + (R(1) - R(2) * q) * f[I];
             }
             f[D] = v;
         }
-        wall_interpolated<Q, ST, D + 1>(p, nb, nmiss, bq, f);
+        wall_interpolated<Q, ST, D + 1>(p, tb, lo, nmiss, bq, f);
     }
 }
 
 template <int Q, class ST, bool NT, int D = 0>
-__device__ __forceinline__ void collide_store(const StepParams& p, uint32_t nb, typename ST::R rho,
+__device__ __forceinline__ void collide_store(const StepParams& p, int64_t tb, uint32_t lo, typename ST::R rho,
                                               const typename ST::R (&u)[3],
                                               const typename ST::R (&f)[Q], typename ST::R omega) {
     if constexpr (D < Q) {
@@ -143,10 +138,10 @@ __device__ __forceinline__ void collide_store(const StepParams& p, uint32_t nb, 
         using S = typename ST::S;
         const R feq = feq_dir<Q, D>(rho, u);
         const R fo = f[D] + omega * (feq - f[D]);
-        S* dst = reinterpret_cast<S*>(static_cast<char*>(p.B) + D * p.ns * (int64_t)sizeof(S) + nb);
+        S* dst = at<S>(p.B, D * p.poff_plane + tb, lo);
         if constexpr (NT) __builtin_nontemporal_store(ST::store(fo, Lat<Q>::w[D]), dst);
         else *dst = ST::store(fo, Lat<Q>::w[D]);
-        collide_store<Q, ST, NT, D + 1>(p, nb, rho, u, f, omega);
+        collide_store<Q, ST, NT, D + 1>(p, tb, lo, rho, u, f, omega);
     }
 }
 
@@ -162,7 +157,37 @@ __device__ __forceinline__ void init_populations(const StepParams& p, int64_t n,
     }
 }
 
-template <int Q, class ST, bool WSS, bool NT>
+// Collision and store fused with the WSS sums of Simulation::ComputeWallShearStress
+// (Simulation.cpp:882-892): for each component I the reference sums, over directions A in index
+// order, fne_A * sum_j(A, I) with fne_A = f_A - f_eq,A(M).  The three sums are carried through the
+// store loop together (each still in its own A order, so every sum rounds exactly as the
+// reference's loop nest does), and f_A dies once its relaxed value is stored, so no Q-long array
+// stays live next to f.
+template <int Q, class ST, bool NT, int D = 0>
+__device__ __forceinline__ void collide_store_wss(const StepParams& p, int64_t tb, uint32_t lo, typename ST::R rho,
+                                                  const typename ST::R (&u)[3], const typename ST::R (&f)[Q],
+                                                  typename ST::R omega, typename ST::R mrho,
+                                                  const typename ST::R (&mu)[3], const typename ST::R (&n)[3],
+                                                  typename ST::R& s0, typename ST::R& s1, typename ST::R& s2) {
+    if constexpr (D < Q) {
+        using R = typename ST::R;
+        using S = typename ST::S;
+        const R feq = feq_dir<Q, D>(rho, u);
+        const R fo = f[D] + omega * (feq - f[D]);
+        S* dst = at<S>(p.B, D * p.poff_plane + tb, lo);
+        if constexpr (NT) __builtin_nontemporal_store(ST::store(fo, Lat<Q>::w[D]), dst);
+        else *dst = ST::store(fo, Lat<Q>::w[D]);
+        const R fne = f[D] - feq_dir<Q, D>(mrho, mu);
+        s0 += fne * wss_weight<Q, R, 0, D>(n);
+        s1 += fne * wss_weight<Q, R, 1, D>(n);
+        s2 += fne * wss_weight<Q, R, 2, D>(n);
+        collide_store_wss<Q, ST, NT, D + 1>(p, tb, lo, rho, u, f, omega, mrho, mu, n, s0, s1, s2);
+    }
+}
+
+// WI: the wall nodes take the interpolated closure (a separate instance, so that the default
+// reference closure does not carry its registers).
+template <int Q, class ST, bool WSS, bool NT, bool WI>
 __device__ __forceinline__ bool node_update(const StepParams& p, double (&red)[4]) {
     using R = typename ST::R;
     using S = typename ST::S;
@@ -174,7 +199,8 @@ __device__ __forceinline__ bool node_update(const StepParams& p, double (&red)[4
     const uint64_t bit = 1ull << lane;
     if (!(T.node & bit)) return false;
     const int64_t n = (int64_t)T.tile * 64 + lane;
-    const uint32_t nb = (uint32_t)((n + p.guard) * (int64_t)sizeof(S));
+    const int64_t tb = ((int64_t)T.tile * 64 + p.guard) * (int64_t)sizeof(S);
+    const uint32_t lo = (uint32_t)lane * (uint32_t)sizeof(S);
     const bool init = (p.flags & F_INIT) != 0;
     const bool bnd = (T.bnd & bit) != 0;
     const int32_t bslot = T.bslot + lanes_below(T.bnd);
@@ -184,7 +210,7 @@ __device__ __forceinline__ bool node_update(const StepParams& p, double (&red)[4
     R f[Q];
     if (!init) {
         S v[Q];
-        load_all<Q, ST>(p, nb, v);
+        load_all<Q, ST>(p, tb, lo, v);
 #pragma unroll
         for (int d = 0; d < Q; d++) f[d] = ST::load(v[d], Lat<Q>::w[d]);
         if constexpr (ST::kGate) {
@@ -199,8 +225,8 @@ __device__ __forceinline__ bool node_update(const StepParams& p, double (&red)[4
             // closures in the reference's call order (Simulation.cpp:582-584); one per node
             cl = p.bclose[bslot];
             if (cl == BC_WALL) {
-                if (p.bq)
-                    wall_interpolated<Q, ST>(p, nb, nmiss, p.bq + (int64_t)bslot * Q, f);
+                if constexpr (WI)
+                    wall_interpolated<Q, ST>(p, tb, lo, nmiss, p.bq + (int64_t)bslot * Q, f);
                 else
                     wall_rule<Q>(f, nmiss);
             } else if (cl == BC_INLET_XMIN) {
@@ -225,52 +251,56 @@ __device__ __forceinline__ bool node_update(const StepParams& p, double (&red)[4
             R fw[Q];
 #pragma unroll
             for (int d = 0; d < Q; d++) fw[d] = R(Lat<Q>::w[d]);
-            save_stale<Q, ST>(p, nb, nmiss, fw);
+            save_stale<Q, ST>(p, tb, lo, nmiss, fw);
         } else {
-            save_stale<Q, ST>(p, nb, nmiss, f);
+            save_stale<Q, ST>(p, tb, lo, nmiss, f);
         }
     }
 
     R rho, u[3];
     moments(f, rho, u);
 
-    if (T.ntw & bit) {
-        const int32_t ws = T.wslot + lanes_below(T.ntw);
-        R* M = static_cast<R*>(p.M);
-        const int64_t ms = p.mst;
-        if constexpr (WSS) {
-            if (!init) {
-                R mu[3] = {M[ms + ws], M[2 * ms + ws], M[3 * ms + ws]};
-                R feq_old[Q];
-                equilibrium<Q>(M[ws], mu, feq_old);
-                R fne[Q];
-#pragma unroll
-                for (int d = 0; d < Q; d++) fne[d] = f[d] - feq_old[d];
-                const R nv[3] = {R(p.nrm[ws]), R(p.nrm[ms + ws]), R(p.nrm[2 * ms + ws])};
-                R s0 = R(0.0), s1 = R(0.0), s2 = R(0.0);
-                wss_sum<Q, R, 0>(fne, nv, s0);
-                wss_sum<Q, R, 1>(fne, nv, s1);
-                wss_sum<Q, R, 2>(fne, nv, s2);
-                const R C = R(p.C);
-                const R t0 = C * s0, t1 = C * s1, t2 = C * s2;
-                const R W = sqrt(t0 * t0 + t1 * t1 + t2 * t2);
-                p.wss[ws] = (double)W;
-                p.tau3[ws] = (double)t0;
-                p.tau3[ms + ws] = (double)t1;
-                p.tau3[2 * ms + ws] = (double)t2;
-                p.acc[ws] += (double)t0;
-                p.acc[ms + ws] += (double)t1;
-                p.acc[2 * ms + ws] += (double)t2;
-                p.acc[3 * ms + ws] += (double)W;
-            }
+    // WSS launches run every lane through the fused pass (the wave executes it if any lane is
+    // next to the wall); lanes that are not next-to-wall take M = rest, n = 0 and discard the sums.
+    const bool ntw = (T.ntw & bit) != 0;
+    const int32_t ws = T.wslot + lanes_below(T.ntw);
+    R* M = static_cast<R*>(p.M);
+    const int64_t ms = p.mst;
+    if (WSS && !init) {
+        R mr = R(1), mu[3] = {R(0), R(0), R(0)}, nv[3] = {R(0), R(0), R(0)};
+        if (ntw) {
+            mr = M[ws];
+            mu[0] = M[ms + ws];
+            mu[1] = M[2 * ms + ws];
+            mu[2] = M[3 * ms + ws];
+            nv[0] = R(p.nrm[ws]);
+            nv[1] = R(p.nrm[ms + ws]);
+            nv[2] = R(p.nrm[2 * ms + ws]);
         }
+        R s0 = R(0.0), s1 = R(0.0), s2 = R(0.0);
+        collide_store_wss<Q, ST, NT>(p, tb, lo, rho, u, f, R(p.omega), mr, mu, nv, s0, s1, s2);
+        if (ntw) {
+            const R C = R(p.C);
+            const R t0 = C * s0, t1 = C * s1, t2 = C * s2;
+            const R W = sqrt(t0 * t0 + t1 * t1 + t2 * t2);
+            p.wss[ws] = (double)W;
+            p.tau3[ws] = (double)t0;
+            p.tau3[ms + ws] = (double)t1;
+            p.tau3[2 * ms + ws] = (double)t2;
+            p.acc[ws] += (double)t0;
+            p.acc[ms + ws] += (double)t1;
+            p.acc[2 * ms + ws] += (double)t2;
+            p.acc[3 * ms + ws] += (double)W;
+        }
+    } else {
+        collide_store<Q, ST, NT>(p, tb, lo, rho, u, f, R(p.omega));
+    }
+    if (ntw) {
         M[ws] = rho;
         M[ms + ws] = u[0];
         M[2 * ms + ws] = u[1];
         M[3 * ms + ws] = u[2];
     }
-
-    collide_store<Q, ST, NT>(p, nb, rho, u, f, R(p.omega));
 
     if (p.flags & F_MACRO) {
         p.mrho[n] = (double)rho;
@@ -290,10 +320,10 @@ __device__ __forceinline__ bool node_update(const StepParams& p, double (&red)[4
     return true;
 }
 
-template <int Q, class ST, bool WSS, bool NT>
+template <int Q, class ST, bool WSS, bool NT, bool WI>
 __global__ __launch_bounds__(256) void k_step(const StepParams p) {
     double red[4] = {0.0, 0.0, 0.0, 0.0};
-    node_update<Q, ST, WSS, NT>(p, red);
+    node_update<Q, ST, WSS, NT, WI>(p, red);
     if (p.flags & F_REDUCE) {
         // FP64 block sums of (mass, inlet flux, outlet flux, non-finite count)
         __shared__ double s[4][4];
@@ -327,18 +357,24 @@ __global__ __launch_bounds__(256) void k_reduce_final(const double* partials, in
     if (threadIdx.x < 4) out4[threadIdx.x] = s[threadIdx.x][0];
 }
 
-template <int Q, class ST>
-void launch_q(const StepParams& p, hipStream_t s) {
+template <int Q, class ST, bool WI>
+void launch_qw(const StepParams& p, hipStream_t s) {
     const dim3 g(p.blocks), b(256);
     // nontemporal stores keep a lattice larger than the caches from evicting the pulls; a
     // per-step working set that fits the last-level cache is served from it with plain stores
     if (p.stream_stores) {
-        if (p.flags & F_WSS) hipLaunchKernelGGL((k_step<Q, ST, true, true>), g, b, 0, s, p);
-        else hipLaunchKernelGGL((k_step<Q, ST, false, true>), g, b, 0, s, p);
+        if (p.flags & F_WSS) hipLaunchKernelGGL((k_step<Q, ST, true, true, WI>), g, b, 0, s, p);
+        else hipLaunchKernelGGL((k_step<Q, ST, false, true, WI>), g, b, 0, s, p);
     } else {
-        if (p.flags & F_WSS) hipLaunchKernelGGL((k_step<Q, ST, true, false>), g, b, 0, s, p);
-        else hipLaunchKernelGGL((k_step<Q, ST, false, false>), g, b, 0, s, p);
+        if (p.flags & F_WSS) hipLaunchKernelGGL((k_step<Q, ST, true, false, WI>), g, b, 0, s, p);
+        else hipLaunchKernelGGL((k_step<Q, ST, false, false, WI>), g, b, 0, s, p);
     }
+}
+
+template <int Q, class ST>
+void launch_q(const StepParams& p, hipStream_t s) {
+    if (p.bq) launch_qw<Q, ST, true>(p, s);
+    else launch_qw<Q, ST, false>(p, s);
 }
 
 }  // namespace

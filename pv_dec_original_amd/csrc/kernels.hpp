@@ -37,6 +37,12 @@ struct StepParams {
     int64_t ns;         // elements per direction plane: guard + padded node count + guard
     int64_t guard;      // guard elements before node 0 of each plane (> largest stencil offset)
     int64_t sy, sz;     // index strides of y and z (x stride 1)
+    // Byte offsets from a lattice base: plane d starts at poff_plane * d; the pull of direction d
+    // (and the stale slot it reads) sits poff_pull[d] from the base plus the node's byte index, so
+    // a wave addresses every plane from one 64-bit scalar base per direction and one 32-bit lane
+    // offset, with no 32-bit limit on the size of a plane.
+    int64_t poff_plane;
+    int64_t poff_pull[27];
     const TileDesc* tdesc;  // active tiles
     int32_t ntiles;
     int32_t blocks;         // launched blocks (4 tiles each)

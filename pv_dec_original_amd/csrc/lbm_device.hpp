@@ -218,24 +218,22 @@ __device__ __forceinline__ void wall_rule(R (&m)[Q], uint32_t nmiss) {
     }
 }
 
-// Wall shear stress of one node, Simulation::ComputeWallShearStress (Simulation.cpp:882-892),
-// the loops unrolled over compile-time directions with the same operation order.
-template <int Q, class R, int I, int A = 0>
-__device__ __forceinline__ void wss_sum(const R (&fne)[Q], const R (&n)[3], R& sum_alpha) {
-    if constexpr (A < Q) {
-        constexpr int c0 = Lat<Q>::cx[A], c1 = Lat<Q>::cy[A], c2 = Lat<Q>::cz[A];
-        constexpr int ci = (I == 0) ? c0 : (I == 1) ? c1 : c2;
-        R sum_k = R(0.0);
-        sum_k += R(c0) * n[0] * n[I];
-        sum_k += R(c1) * n[1] * n[I];
-        sum_k += R(c2) * n[2] * n[I];
-        R sum_j = R(0.0);
-        sum_j += R(c0) * n[0] * (R(ci) - sum_k);
-        sum_j += R(c1) * n[1] * (R(ci) - sum_k);
-        sum_j += R(c2) * n[2] * (R(ci) - sum_k);
-        sum_alpha += fne[A] * sum_j;
-        wss_sum<Q, R, I, A + 1>(fne, n, sum_alpha);
-    }
+// Wall shear stress of one node, Simulation::ComputeWallShearStress (Simulation.cpp:882-892):
+// the weight sum_j(A, I) = sum_k c_A,k n_k (c_A,I - sum_l c_A,l n_l n_I) that multiplies the
+// non-equilibrium population of direction A in component I, with the loop's operation order.
+template <int Q, class R, int I, int A>
+__device__ __forceinline__ R wss_weight(const R (&n)[3]) {
+    constexpr int c0 = Lat<Q>::cx[A], c1 = Lat<Q>::cy[A], c2 = Lat<Q>::cz[A];
+    constexpr int ci = (I == 0) ? c0 : (I == 1) ? c1 : c2;
+    R sum_k = R(0.0);
+    sum_k += R(c0) * n[0] * n[I];
+    sum_k += R(c1) * n[1] * n[I];
+    sum_k += R(c2) * n[2] * n[I];
+    R sum_j = R(0.0);
+    sum_j += R(c0) * n[0] * (R(ci) - sum_k);
+    sum_j += R(c1) * n[1] * (R(ci) - sum_k);
+    sum_j += R(c2) * n[2] * (R(ci) - sum_k);
+    return sum_j;
 }
 
 }  // namespace pvlbm
