@@ -43,6 +43,14 @@ int pvgen_aneurysm(uint64_t seed, double nx, double ny, double nz, double rt, do
                    double xcap0, double xcap1, const double lo[3], const double hi[3],
                    double spacing, pvgen_mesh** out);
 
+/* Vessel tree of straight segments: seg[11 * s + ...] = a (3), b (3), ra, rb, end_axis, end_val,
+ * end_keep.  Segment s is the capsule around a -> b whose radius runs linearly from ra to rb; with
+ * end_axis in 0..2 it is cut flat by the plane p[end_axis] = end_val, keeping p[end_axis] >= end_val
+ * (end_keep > 0) or <= end_val (end_keep < 0); end_axis = -1 leaves the rounded end.  The union
+ * is cut by the box [box_lo, box_hi] and meshed as pvgen_ybifurcation's shape is. */
+int pvgen_tree(int nseg, const double* seg, const double box_lo[3], const double box_hi[3],
+               const double lo[3], const double hi[3], double spacing, pvgen_mesh** out);
+
 int64_t pvgen_mesh_size(const pvgen_mesh* m);
 /* copy the triangles, 12 doubles each */
 void pvgen_mesh_copy(const pvgen_mesh* m, double* tri);

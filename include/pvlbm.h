@@ -80,10 +80,13 @@ enum pvlbm_list_id {
 
 /* A face specification: the reference hard-codes four planes of PV.stl
  * (ComplexGeometry.cpp:1378-1408); this generalises each to {axis, plane, tolerance, window}.
- * A boundary node at (c0, c1, c2) matches when |c[axis] - plane| <= tol and, for the two other
- * axes a < b in increasing order, lo[0] < c[a] < hi[0] and lo[1] < c[b] < hi[1], compared in
- * long double as the reference does.  Matching is tried in the order inlet, outlet1, outlet2,
- * outlet3 (Subdomain.cpp:124-155); a boundary node that matches nothing is a wall node. */
+ * A boundary node at (c0, c1, c2) matches when |c[axis] - P| <= tol and, for the two other
+ * axes a < b in increasing order, lo[0] < c[a] < hi[0] and lo[1] < c[b] < hi[1] (open intervals),
+ * compared in long double as the reference does, with P = (long double)plane + (long double)plane_lo:
+ * the reference's planes are long double literals (-16.9538L, -170.016L), which a double pair
+ * carries exactly and a single double does not (plane_lo = 0 for a plane that is a double).
+ * Matching is tried in the order inlet, outlet1, outlet2, outlet3, and among the faces of one role
+ * in array order (Subdomain.cpp:124-155); a boundary node that matches nothing is a wall node. */
 typedef struct pvlbm_face {
     int32_t role;    /* enum pvlbm_role */
     int32_t closure; /* enum pvlbm_closure */
@@ -94,6 +97,7 @@ typedef struct pvlbm_face {
     double lo[2];
     double hi[2];
     double rho;      /* prescribed density of a pressure outlet (the reference uses 1.0) */
+    double plane_lo; /* low part of the plane coordinate (see above); 0 for a double plane */
 } pvlbm_face;
 
 /* A lattice grid: node (i,j,k) sits at origin + (i,j,k)*h. */

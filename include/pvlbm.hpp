@@ -10,6 +10,7 @@
 #ifndef PVLBM_HPP
 #define PVLBM_HPP
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -199,29 +200,52 @@ public:
     void step(int n, const double* inlet_scale = nullptr) { check(pvlbm_step(h_, n, inlet_scale), h_); }
     static double getInletVelocity(double t, double period) { return pvlbm_pulsatile_velocity(t, period); }
 
-    /* Simulation::Iterate (Simulation.cpp:569-601): while total_runtime <= t_end, one iteration
-     * with the pulsatile inlet at total_runtime, total_runtime += dt.  every_iter, when set, is
-     * called after each iteration whose index is a multiple of output_every. */
+    /* Iterations 0 .. n-1 of the loop body with InletVelocity_ = scale[iter].  output(iter) is
+     * called after every iteration with iter % output_every == 0, iteration 0 included (WriteOutput,
+     * Simulation.cpp:589-591); inside it surfaceAttributes() holds that iteration's WSS, OSI and
+     * TAWSS and, when fields is true, outputRho() / outputU() hold the density and velocity its
+     * Collision computed (Simulation.cpp:52-105).  One engine launch finishes an iteration and runs
+     * the next one's collision, so those fields are read before the launch that finishes it. */
     template <class F>
-    void Iterate(double t_end, double period, int output_every, F every_iter) {
-        total_runtime = 0.0;
-        int64_t it = 0;
-        std::vector<double> sc;
-        while (total_runtime <= t_end) {
-            sc.clear();
-            int64_t n = 0;
-            double t = total_runtime;
-            while (t <= t_end && n < output_every - (it % output_every)) {
-                sc.push_back(getInletVelocity(t, period));
-                t += m_dt;
-                n++;
+    void Run(const std::vector<double>& scale, int output_every, F output, bool fields = false) {
+        if (output_every < 1) throw std::runtime_error("output_every must be >= 1");
+        const int64_t n = (int64_t)scale.size();
+        for (int64_t it = 0; it < n;) {
+            // iterations it .. last, ending at the next output iteration
+            const int64_t next = (it + output_every - 1) / output_every * output_every;
+            const int64_t last = std::min(n - 1, next);
+            const bool out = last % output_every == 0;
+            const int64_t m = last - it + 1;
+            if (out && fields) {
+                if (m > 1) step((int)(m - 1), scale.data() + it);
+                macros(out_rho_, out_u_);
+                step(1, scale.data() + last);
+            } else {
+                step((int)m, scale.data() + it);
             }
-            step((int)n, sc.data());
-            it += n;
-            total_runtime = t;
-            if (it % output_every == 0 || total_runtime > t_end) every_iter(it - 1);
+            it = last + 1;
+            if (out) output(last);
         }
     }
+
+    /* Simulation::Iterate (Simulation.cpp:569-601): total_runtime starts at 0; while
+     * total_runtime <= t_end, one iteration with the pulsatile inlet at total_runtime, then
+     * total_runtime += dt (an accumulated sum, as the reference does); outputs as Run. */
+    template <class F>
+    void Iterate(double t_end, double period, int output_every, F output, bool fields = false) {
+        std::vector<double> sc;
+        double t = 0.0;
+        while (t <= t_end) {
+            sc.push_back(getInletVelocity(t, period));
+            t += m_dt;
+        }
+        Run(sc, output_every, output, fields);
+        total_runtime = t;
+    }
+
+    // density [nz][ny][nx] and velocity (three such planes) written at the last output of Run
+    const std::vector<double>& outputRho() const { return out_rho_; }
+    const std::vector<double>& outputU() const { return out_u_; }
 
     /* x, y, z, WSS, OSI, TAWSS per next-to-wall node (OutputFile.cpp:884-894) */
     // (rho, u) of every grid point, rho [nz][ny][nx] then u as three such planes (x, y, z)
@@ -252,6 +276,7 @@ private:
     double pressure_;  // stored; the outlets use their face density (Simulation.cpp:630)
     double m_dt;
     double m_relaxation = 1.0;
+    std::vector<double> out_rho_, out_u_;
 };
 
 }  // namespace pvlbm

@@ -555,28 +555,50 @@ bool face_match(const pvlbm_face& F, long double c[3]) {
     long double tolerance = F.tol;
     int a = (F.axis == 0) ? 1 : 0;
     int b = (F.axis == 2) ? 1 : 2;
-    long double plane = F.plane;
+    long double plane = (long double)F.plane + (long double)F.plane_lo;
     return (std::abs(c[F.axis] - plane) <= tolerance) && (F.lo[0] < c[a] && c[a] < F.hi[0]) &&
            (F.lo[1] < c[b] && c[b] < F.hi[1]);
 }
 
+}  // namespace
+
+// classifyBoundaryPoints (ComplexGeometry.cpp:1369-1409) and the precedence of CreateNodes
+// (Subdomain.cpp:124-155) for one boundary point: the index of the face it is assigned to, or -1
+// (a wall point).  The reference pushes a point onto every list whose plane and window it matches;
+// CreateNodes then tries the inlet list first, then outlets 1, 2 and 3, so the first role that
+// matches wins (and here, within a role, the first face in array order).
+extern "C" int orc_classify_point(const pvlbm_face* faces, int nfaces, const double c[3]) {
+    long double c3[3] = {(long double)c[0], (long double)c[1], (long double)c[2]};
+    for (int role = PVLBM_INLET; role <= PVLBM_OUTLET3; role++)
+        for (int fi = 0; fi < nfaces; fi++)
+            if (faces[fi].role == role && face_match(faces[fi], c3)) return fi;
+    return -1;
+}
+
+namespace {
+
 // Simulation::ComputeWallShearStress for one node (Simulation.cpp:880-898).
-void wss_node(const orc_sim* s, const double* f, const double* feq, const double* n,
-              double C, double tau_i[3]) {
+void wss_node(const Lat& L, const double* f, const double* feq, const double* n, double C,
+              double tau_i[3]) {
     int dim = 3;
     for (int i = 0; i < dim; i++) {
         double sum_alpha = 0.0;
-        for (int alpha = 0; alpha < s->q; alpha++) {
+        for (int alpha = 0; alpha < L.q; alpha++) {
             double sum_k = 0.0;
-            for (int k = 0; k < dim; k++) sum_k += (double)s->L.c[alpha][k] * n[k] * n[i];
+            for (int k = 0; k < dim; k++) sum_k += (double)L.c[alpha][k] * n[k] * n[i];
             double sum_j = 0.0;
             for (int j = 0; j < dim; j++)
-                sum_j += (double)s->L.c[alpha][j] * n[j] * ((double)s->L.c[alpha][i] - sum_k);
+                sum_j += (double)L.c[alpha][j] * n[j] * ((double)L.c[alpha][i] - sum_k);
             double f_ne_alpha = f[alpha] - feq[alpha];
             sum_alpha += f_ne_alpha * sum_j;
         }
         tau_i[i] = C * sum_alpha;
     }
+}
+
+void wss_node(const orc_sim* s, const double* f, const double* feq, const double* n,
+              double C, double tau_i[3]) {
+    wss_node(s->L, f, feq, n, C, tau_i);
 }
 
 double wss_constant(const orc_sim* s) {
@@ -680,16 +702,8 @@ extern "C" orc_sim* orc_sim_create(const uint8_t* fluid, const int32_t dims[3],
             }
         }
         if (!boundary) continue;
-        // classifyBoundaryPoints (:1369-1409) + CreateNodes precedence (Subdomain.cpp:124-155)
-        long double c3[3] = {(long double)(s->ox + (double)i * h), (long double)(s->oy + (double)j * h),
-                             (long double)(s->oz + (double)k * h)};
-        int matched = -1;
-        for (int role = PVLBM_INLET; role <= PVLBM_OUTLET3 && matched < 0; role++)
-            for (int fi = 0; fi < nfaces; fi++)
-                if (faces[fi].role == role && face_match(faces[fi], c3)) {
-                    matched = fi;
-                    break;
-                }
+        const double c3[3] = {s->ox + (double)i * h, s->oy + (double)j * h, s->oz + (double)k * h};
+        const int matched = orc_classify_point(faces, nfaces, c3);
         if (matched >= 0) {
             s->kind[n] = faces[matched].role == PVLBM_INLET ? PVLBM_KINLET : PVLBM_KOUTLET;
             s->face[n] = (int8_t)matched;
@@ -1080,6 +1094,11 @@ extern "C" void orc_collide(int q, double tau, const double* f, double* feq, dou
     out4[1] = u[0];
     out4[2] = u[1];
     out4[3] = u[2];
+}
+
+extern "C" void orc_wss_node(int q, const double* f, const double* feq, const double n[3], double C,
+                             double tau[3]) {
+    wss_node(lattice(q), f, feq, n, C, tau);
 }
 
 extern "C" void orc_zouhe(int q, int closure, double* f, double ux, double rho_out) {

@@ -47,6 +47,12 @@ void orc_label_points_binned(const double* tri, int64_t ntri, const double bb[6]
                              const int32_t dims[3], const double origin[3], double h,
                              uint8_t* fluid, int threads);
 
+/* classifyBoundaryPoints and the CreateNodes precedence (ComplexGeometry.cpp:1369-1409,
+ * Subdomain.cpp:124-155) for one boundary point at c: the index of the face it is assigned to
+ * (first matching role in the order inlet, outlet 1, 2, 3; first face of that role in array
+ * order), or -1 for a wall point. */
+int orc_classify_point(const pvlbm_face* faces, int nfaces, const double c[3]);
+
 typedef struct orc_sim orc_sim;
 
 /* SubDomain construction (Subdomain.cpp:3-32, 91-218, 224-370, 549-596) from voxel flags, with
@@ -98,6 +104,9 @@ void orc_sim_link_fractions(const orc_sim* s, const double* tri, int64_t ntri, i
  * feq = its equilibrium, fnew = the relaxed populations. */
 void orc_collide(int q, double tau, const double* f, double* feq, double* fnew, double* out4);
 /* Zou-He closures on one node's populations (for closure tests). */
+/* Wall shear stress vector of one node (Simulation::ComputeWallShearStress, Simulation.cpp:880-898):
+ * tau[i] = C sum_alpha (f - feq)_alpha sum_j c_alpha,j n_j (c_alpha,i - sum_k c_alpha,k n_k n_i). */
+void orc_wss_node(int q, const double* f, const double* feq, const double n[3], double C, double tau[3]);
 void orc_zouhe(int q, int closure, double* f, double ux, double rho_out);
 /* Simulation::getInletVelocity (Simulation.cpp:548-563). */
 double orc_pulsatile_velocity(double t, double period);

@@ -32,10 +32,11 @@ _i64 = C.POINTER(C.c_int64)
 
 
 class Face(C.Structure):
-    """pvlbm_face: {role, closure, axis, reserved, plane, tol, lo[2], hi[2], rho}."""
+    """pvlbm_face: {role, closure, axis, reserved, plane, tol, lo[2], hi[2], rho, plane_lo}."""
     _fields_ = [("role", C.c_int32), ("closure", C.c_int32), ("axis", C.c_int32),
                 ("reserved", C.c_int32), ("plane", C.c_double), ("tol", C.c_double),
-                ("lo", C.c_double * 2), ("hi", C.c_double * 2), ("rho", C.c_double)]
+                ("lo", C.c_double * 2), ("hi", C.c_double * 2), ("rho", C.c_double),
+                ("plane_lo", C.c_double)]
 
 
 class Grid(C.Structure):
@@ -149,6 +150,7 @@ PVGEN_API = [
     ("pvgen_aneurysm", C.c_int, [C.c_uint64, C.c_double, C.c_double, C.c_double, C.c_double,
                                  C.c_double, C.c_double, C.c_double, _d, _d, C.c_double,
                                  C.POINTER(C.c_void_p)]),
+    ("pvgen_tree", C.c_int, [C.c_int, _d, _d, _d, _d, _d, C.c_double, C.POINTER(C.c_void_p)]),
     ("pvgen_mesh_size", C.c_int64, [C.c_void_p]),
     ("pvgen_mesh_copy", None, [C.c_void_p, _d]),
     ("pvgen_mesh_free", None, [C.c_void_p]),

@@ -258,7 +258,7 @@ int guarded(pvlbm_sim* h, F&& fn) {
 bool face_match(const pvlbm_face& F, const long double c[3]) {
     // ComplexGeometry.cpp:1380-1407: |c - plane| <= tol and open windows, in long double
     long double tolerance = F.tol;
-    long double plane = F.plane;
+    long double plane = (long double)F.plane + (long double)F.plane_lo;
     int a = (F.axis == 0) ? 1 : 0;
     int b = (F.axis == 2) ? 1 : 2;
     return (std::abs(c[F.axis] - plane) <= tolerance) && (F.lo[0] < c[a] && c[a] < F.hi[0]) &&

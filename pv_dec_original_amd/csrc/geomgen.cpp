@@ -59,6 +59,10 @@ struct Rng {
 
 struct Seg {
     double a[3], b[3], ra, rb;
+    // optional flat end: the segment keeps p[end_axis] >= end_val (end_keep > 0) or <= (< 0)
+    int end_axis = -1;
+    double end_val = 0.0;
+    int end_keep = 1;
 };
 
 struct Grid {
@@ -87,7 +91,12 @@ double seg_field(const double p[3], const Seg& s) {
     double t = ll > 0 ? (ap[0] * ab[0] + ap[1] * ab[1] + ap[2] * ab[2]) / ll : 0.0;
     t = std::min(1.0, std::max(0.0, t));
     double d[3] = {ap[0] - t * ab[0], ap[1] - t * ab[1], ap[2] - t * ab[2]};
-    return std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) - (s.ra + t * (s.rb - s.ra));
+    double v = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) - (s.ra + t * (s.rb - s.ra));
+    if (s.end_axis >= 0) {
+        const double cut = s.end_keep > 0 ? s.end_val - p[s.end_axis] : p[s.end_axis] - s.end_val;
+        v = std::max(v, cut);
+    }
+    return v;
 }
 
 // phi = min(phi, field of each segment), evaluated only inside each segment's padded bbox
@@ -123,6 +132,18 @@ void cut_slab(Grid& G, double x0, double x1) {
                 double x = G.pos(0, i);
                 double& v = G.phi[G.id(i, j, k)];
                 v = std::max(v, std::max(x0 - x, x - x1));
+            }
+}
+
+// keep the solid inside the box [blo, bhi] on every axis
+void cut_box(Grid& G, const double blo[3], const double bhi[3]) {
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < G.g[2]; k++)
+        for (int64_t j = 0; j < G.g[1]; j++)
+            for (int64_t i = 0; i < G.g[0]; i++) {
+                const double p[3] = {G.pos(0, i), G.pos(1, j), G.pos(2, k)};
+                double& v = G.phi[G.id(i, j, k)];
+                for (int a = 0; a < 3; a++) v = std::max(v, std::max(blo[a] - p[a], p[a] - bhi[a]));
             }
 }
 
@@ -392,6 +413,33 @@ extern "C" int pvgen_aneurysm(uint64_t seed, double nx, double ny, double nz, do
     double c[3] = {xs, ys, cz + rt + 0.45 * rs};
     add_sphere(G, c, rs);
     return finish(G, xcap0, xcap1, out);
+}
+
+extern "C" int pvgen_tree(int nseg, const double* seg, const double box_lo[3], const double box_hi[3],
+                          const double lo[3], const double hi[3], double spacing, pvgen_mesh** out) {
+    if (!out || spacing <= 0 || nseg <= 0 || !seg) return 1;
+    std::vector<Seg> segs((size_t)nseg);
+    for (int i = 0; i < nseg; i++) {
+        const double* v = seg + 11 * i;
+        Seg& S = segs[(size_t)i];
+        for (int a = 0; a < 3; a++) {
+            S.a[a] = v[a];
+            S.b[a] = v[3 + a];
+        }
+        S.ra = v[6];
+        S.rb = v[7];
+        S.end_axis = (int)v[8];
+        S.end_val = v[9];
+        S.end_keep = v[10] < 0 ? -1 : 1;
+        if (S.end_axis > 2 || S.ra <= 0 || S.rb <= 0) return 1;
+    }
+    Grid G = make_grid(lo, hi, spacing);
+    splat_segments(G, segs);
+    cut_box(G, box_lo, box_hi);
+    pvgen_mesh* m = new pvgen_mesh();
+    contour(G, m->tri);
+    *out = m;
+    return 0;
 }
 
 extern "C" int64_t pvgen_mesh_size(const pvgen_mesh* m) { return m ? (int64_t)(m->tri.size() / 12) : 0; }

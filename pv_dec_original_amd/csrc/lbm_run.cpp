@@ -12,10 +12,11 @@
 //
 // --faces pv uses the reference's PV.stl planes (ComplexGeometry.cpp:1378-1408); --faces x puts the
 // inlet on the grid's x-min plane and a pressure outlet (rho 1) on its x-max plane.  Without
-// --steps the run is the reference loop: the pulsatile waveform until --t-end.  Every
-// --output-every iterations FlowHemodynamics/<iter>_FlowHemodynamics.csv is written in the
-// reference's format (OutputFile.cpp:867-938); with --vtu also Fluid_Variables/<n>.vtu, n counting
-// the outputs from 0 (OutputFile.cpp:259-429).
+// --steps the run is the reference loop: the pulsatile waveform until --t-end.  After every
+// iteration whose index is a multiple of --output-every, iteration 0 included (Simulation.cpp:589-591),
+// FlowHemodynamics/<iter>_FlowHemodynamics.csv is written in the reference's format
+// (OutputFile.cpp:867-938); with --vtu also Fluid_Variables/<iter>.vtu with the density and velocity
+// of that iteration's collision (OutputFile.cpp:259-429, named as at :275).
 
 #include <sys/stat.h>
 
@@ -42,13 +43,16 @@ std::vector<double> parse_list(const std::string& s) {
     return v;
 }
 
-pvlbm_face make_face(int role, int closure, int axis, double plane, double tol, double lo0, double lo1,
+// plane is a long double so that the reference's long double literals (ComplexGeometry.cpp:1376,
+// 1385) reach the face match exactly, as the pair plane + plane_lo
+pvlbm_face make_face(int role, int closure, int axis, long double plane, double tol, double lo0, double lo1,
                      double hi0, double hi1, double rho) {
     pvlbm_face f{};
     f.role = role;
     f.closure = closure;
     f.axis = axis;
-    f.plane = plane;
+    f.plane = (double)plane;
+    f.plane_lo = (double)(plane - (long double)f.plane);
     f.tol = tol;
     f.lo[0] = lo0;
     f.lo[1] = lo1;
@@ -73,7 +77,7 @@ void write_csv(const std::string& outdir, int64_t iter, const std::vector<double
 // (OutputFile.cpp:259-429): fluid nodes at integer lattice indices in z-y-x loop order, point data
 // Velocity, Density, VelocityMagnitude, a hexahedron per cube of eight fluid corners; XML with raw
 // appended binary blocks (UInt64 byte count, then little-endian data).
-void write_vtu(const std::string& outdir, int idx, const std::vector<uint8_t>& kind, int nx, int ny, int nz,
+void write_vtu(const std::string& outdir, int64_t idx, const std::vector<uint8_t>& kind, int nx, int ny, int nz,
                const std::vector<double>& rho, const std::vector<double>& u) {
     const size_t n = (size_t)nx * ny * nz;
     std::vector<int64_t> pid(n, -1);
@@ -218,10 +222,10 @@ int main(int argc, char** argv) {
         const double big = 1e300;
         if (faces_kind == "pv") {
             // the reference's PV.stl planes (ComplexGeometry.cpp:1378-1408)
-            faces.push_back(make_face(PVLBM_INLET, PVLBM_INLET_XMIN, 0, -16.9538, 1e-2, -191.0, -118.0, -187.0, -114.0, 1.0));
-            faces.push_back(make_face(PVLBM_OUTLET1, PVLBM_OUTLET_YMAX, 1, -170.016, 1e-2, 2.0, -109.0, 5.0, -106.0, 1.0));
-            faces.push_back(make_face(PVLBM_OUTLET2, PVLBM_OUTLET_ZMIN, 2, -110.0, 1e-2, -27.0, -198.0, -25.0, -195.0, 1.0));
-            faces.push_back(make_face(PVLBM_OUTLET3, PVLBM_OUTLET_ZMIN, 2, -123.0, 1e-2, -33.0, -189.0, -30.0, -186.0, 1.0));
+            faces.push_back(make_face(PVLBM_INLET, PVLBM_INLET_XMIN, 0, -16.9538L, 1e-2, -191.0, -118.0, -187.0, -114.0, 1.0));
+            faces.push_back(make_face(PVLBM_OUTLET1, PVLBM_OUTLET_YMAX, 1, -170.016L, 1e-2, 2.0, -109.0, 5.0, -106.0, 1.0));
+            faces.push_back(make_face(PVLBM_OUTLET2, PVLBM_OUTLET_ZMIN, 2, -110.0L, 1e-2, -27.0, -198.0, -25.0, -195.0, 1.0));
+            faces.push_back(make_face(PVLBM_OUTLET3, PVLBM_OUTLET_ZMIN, 2, -123.0L, 1e-2, -33.0, -189.0, -30.0, -186.0, 1.0));
         } else if (faces_kind == "x") {
             const double x0 = geo->grid.origin[0], x1 = x0 + (geo->grid.dims[0] - 1) * spacing;
             faces.push_back(make_face(PVLBM_INLET, PVLBM_INLET_XMIN, 0, x0, 1e-2 * spacing, -big, -big, big, big, 1.0));
@@ -234,20 +238,18 @@ int main(int argc, char** argv) {
         std::cerr << "grid " << geo->grid.dims[0] << "x" << geo->grid.dims[1] << "x" << geo->grid.dims[2]
                   << ", nodes " << sd.count(PVLBM_LIST_FLUID) << ", inlet " << sd.count(PVLBM_LIST_INLET)
                   << ", wall " << sd.count(PVLBM_LIST_WALL) << ", next-to-wall " << sd.count(PVLBM_LIST_NTW) << "\n";
-        int vtu_idx = 0;
         std::vector<uint8_t> kinds;
         if (vtu) kinds = sd.kinds();
+        // WriteOutput (Simulation.cpp:872-876): both files are named by the iteration
         auto out = [&](int64_t it) {
             write_csv(outdir, it, sim.surfaceAttributes());
-            if (vtu) {
-                std::vector<double> rho, u;
-                sim.macros(rho, u);
-                write_vtu(outdir, vtu_idx++, kinds, geo->grid.dims[0], geo->grid.dims[1], geo->grid.dims[2], rho, u);
-            }
+            if (vtu)
+                write_vtu(outdir, it, kinds, geo->grid.dims[0], geo->grid.dims[1], geo->grid.dims[2], sim.outputRho(),
+                          sim.outputU());
             std::cerr << "iteration " << it << " written\n";
         };
         if (steps < 0) {
-            sim.Iterate(t_end, period, output_every, out);
+            sim.Iterate(t_end, period, output_every, out, vtu);
         } else {
             std::vector<double> sc((size_t)steps);
             if (inlet.rfind("const:", 0) == 0) {
@@ -260,12 +262,7 @@ int main(int argc, char** argv) {
             } else {
                 return usage();
             }
-            for (long s = 0; s < steps;) {
-                long m = std::min<long>(output_every - (s % output_every), steps - s);
-                sim.step((int)m, sc.data() + s);
-                s += m;
-                if (s % output_every == 0 || s == steps) out(s - 1);
-            }
+            sim.Run(sc, output_every, out, vtu);
         }
         delete geo;
         double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

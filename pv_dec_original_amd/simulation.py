@@ -20,24 +20,37 @@ from . import _native as N
 BIG = 1e300
 
 
+def split_plane(plane):
+    """(hi, lo) doubles with hi + lo equal to the plane coordinate: a float is (plane, 0); a string
+    is parsed as an x87 long double (numpy.longdouble on x86-64), as the reference's long double
+    literals are (ComplexGeometry.cpp:1376, 1385), and split exactly into two doubles."""
+    if isinstance(plane, str):
+        ld = np.longdouble(plane)
+        hi = float(ld)
+        return hi, float(ld - np.longdouble(hi))
+    return float(plane), 0.0
+
+
 def face(role, closure, axis, plane, tol=1e-2, lo=(-BIG, -BIG), hi=(BIG, BIG), rho=1.0):
     """A face spec: boundary nodes with |c[axis] - plane| <= tol inside the open window
-    lo < (other two coordinates) < hi (ComplexGeometry.cpp:1380-1407 generalised)."""
+    lo < (other two coordinates) < hi (ComplexGeometry.cpp:1380-1407 generalised).  plane may be a
+    string holding a long double literal (see split_plane)."""
     f = N.Face()
     f.role, f.closure, f.axis = role, closure, axis
-    f.plane, f.tol, f.rho = float(plane), float(tol), float(rho)
+    f.plane, f.plane_lo = split_plane(plane)
+    f.tol, f.rho = float(tol), float(rho)
     f.lo[0], f.lo[1] = lo
     f.hi[0], f.hi[1] = hi
     return f
 
 
 def pv_faces():
-    """The reference's hard-coded PV.stl planes (ComplexGeometry.cpp:1378-1408): inlet on
-    x = -16.9538, outlet 1 on y = -170.016 (y-max closure), outlets 2, 3 on z = -110 and z = -123
-    (z-min closure).  The reference compares against long double literals; these are the nearest
-    doubles."""
-    return [face(N.INLET, N.INLET_XMIN, 0, -16.9538, 1e-2, (-191.0, -118.0), (-187.0, -114.0)),
-            face(N.OUTLET1, N.OUTLET_YMAX, 1, -170.016, 1e-2, (2.0, -109.0), (5.0, -106.0)),
+    """The reference's hard-coded PV.stl planes (ComplexGeometry.cpp:1376-1408): inlet on
+    x = -16.9538L, outlet 1 on y = -170.016L (y-max closure), outlets 2, 3 on z = -110 and
+    z = -123 (z-min closure), with their windows and the tolerance 1e-2.  The long double literals
+    are carried exactly as double pairs."""
+    return [face(N.INLET, N.INLET_XMIN, 0, "-16.9538", 1e-2, (-191.0, -118.0), (-187.0, -114.0)),
+            face(N.OUTLET1, N.OUTLET_YMAX, 1, "-170.016", 1e-2, (2.0, -109.0), (5.0, -106.0)),
             face(N.OUTLET2, N.OUTLET_ZMIN, 2, -110.0, 1e-2, (-27.0, -198.0), (-25.0, -195.0)),
             face(N.OUTLET3, N.OUTLET_ZMIN, 2, -123.0, 1e-2, (-33.0, -189.0), (-30.0, -186.0))]
 
@@ -259,31 +272,47 @@ class Simulation:
         """The reference's pulsatile waveform (Simulation.cpp:548-563)."""
         return N.lib().pvlbm_pulsatile_velocity(float(t), float(period))
 
-    def Iterate(self, t_end=2.7, period=0.9, output_every=100, outdir=None, chunk=100):
-        """The reference loop (Simulation.cpp:569-601): while total_runtime <= t_end, one
-        iteration with InletVelocity_ = getInletVelocity(total_runtime, period), then
-        total_runtime += dt (an accumulated sum, as the reference does).  Every output_every
-        iterations the FlowHemodynamics CSV is written when outdir is given."""
+    def Iterate(self, t_end=2.7, period=0.9, output_every=100, outdir=None, vtu=False, chunk=100):
+        """The reference loop (Simulation.cpp:569-601): total_runtime and iter start at 0; while
+        total_runtime <= t_end, iteration iter runs with InletVelocity_ =
+        getInletVelocity(total_runtime, period), then total_runtime += dt (an accumulated sum, as
+        the reference does) and iter += 1.  When outdir is given, WriteOutput(iter) follows every
+        iteration with iter % output_every == 0, iteration 0 included (Simulation.cpp:589-591,
+        872-876): FlowHemodynamics/<iter>_FlowHemodynamics.csv with that iteration's WSS, OSI and
+        TAWSS, and with vtu=True Fluid_Variables/<iter>.vtu with the density and velocity that the
+        iteration's Collision computed (Simulation.cpp:52-105).  One launch of the engine finishes
+        an iteration and runs the next one's collision, so those fields are read before the launch
+        that finishes iteration iter.  Returns the paths written, in order."""
+        self.total_runtime = 0.0
+        it = 0
+        written = []
         while self.total_runtime <= t_end:
-            scales, times = [], []
+            # iterations it .. it + n - 1, ending at the next output iteration or at t_end
+            scales = []
             t = self.total_runtime
-            while len(scales) < chunk and t <= t_end:
+            while t <= t_end and len(scales) < chunk:
                 scales.append(self.getInletVelocity(t, period))
-                times.append(t)
                 t += self.m_dt
+                if outdir is not None and (it + len(scales) - 1) % output_every == 0:
+                    break
             n = len(scales)
-            it0 = self.iteration
-            # step in pieces that end on output iterations
-            done = 0
-            while done < n:
-                it = it0 + done
-                nxt = ((it // output_every) + 1) * output_every if outdir else it0 + n
-                m = min(n - done, nxt - it)
-                self.step(m, scales[done:done + m])
-                done += m
-                if outdir and (it0 + done) % output_every == 0:
-                    self.WriteOutput(it0 + done - 1, outdir)
+            last = it + n - 1
+            out = outdir is not None and last % output_every == 0
+            fields = None
+            if out and vtu:
+                if n > 1:
+                    self.step(n - 1, scales[:n - 1])
+                fields = self.macros()
+                self.step(1, scales[n - 1:])
+            else:
+                self.step(n, scales)
+            if out:
+                written.append(self.WriteOutput(last, outdir))
+                if vtu:
+                    written.append(self.WriteVTK(last, outdir, fields))
+            it += n
             self.total_runtime = t
+        return written
 
     def macros(self):
         """(rho, u) of the current iteration, dense [nz,ny,nx] and [3,nz,ny,nx]."""
@@ -340,14 +369,16 @@ class Simulation:
         self._chk(N.lib().pvlbm_gate_events(self.handle, C.byref(n)))
         return n.value
 
-    def WriteVTK(self, index, outdir):
+    def WriteVTK(self, index, outdir, fields=None):
         """Fluid_Variables/<index>.vtu: velocity, density and speed of every fluid node on the
-        lattice's hexahedra (OutputFile::WriteVTK_Collective, OutputFile.cpp:259-429)."""
+        lattice's hexahedra (OutputFile::WriteVTK_Collective, OutputFile.cpp:259-429; the reference
+        names the file by the iteration, OutputFile.cpp:275).  fields = (rho, u) to write, default
+        the current iteration's macros()."""
         from .output import write_fluid_vtu
         folder = os.path.join(outdir, "Fluid_Variables")
         os.makedirs(folder, exist_ok=True)
         path = os.path.join(folder, f"{index}.vtu")
-        rho, u = self.macros()
+        rho, u = self.macros() if fields is None else fields
         write_fluid_vtu(path, self.SubDomain_.kinds() != N.SOLID, rho, u)
         return path
 

@@ -6,6 +6,8 @@ ybif        symmetric Y-bifurcation (parent R = 24, daughters R = 17 at +-30 deg
             512 x 256 x 256 box; inlet on x-min, both daughters leave through the x-max outlet.
 vessel      seeded tortuous vessel with 1-2 branches in a 96^3 box.
 aneurysm    seeded curved tube R = 40 with a spherical bulge R = 70 in a 1024 x 512 x 512 box.
+branch      a parent tube with branches leaving through +y and -z faces and a flat cap inside the
+            box, with windowed faces (the reference's y-max / z-min outlet closures).
 
 Meshes come from libpvgen.so.  Implicit shapes are contoured on a grid offset from the lattice by
 (-6.37, -6.29, -6.13) so that no vertical lattice ray lies in a contour-cell face.
@@ -63,6 +65,17 @@ def aneurysm_mesh(seed, box=(1024, 512, 512), rt=40.0, rs=70.0, spacing=2.0):
                  -2.5, box[0] + 1.5, N.dptr(lo), N.dptr(hi), spacing)
 
 
+def tree_mesh(segments, box_lo, box_hi, spacing=1.0):
+    """Vessel tree (pvgen_tree): segments = rows of (a, b, ra, rb, end_axis, end_val, end_keep)."""
+    seg = np.ascontiguousarray(np.asarray(segments, dtype=np.float64).reshape(-1, 11))
+    blo = np.ascontiguousarray(box_lo, dtype=np.float64)
+    bhi = np.ascontiguousarray(box_hi, dtype=np.float64)
+    lo = np.ascontiguousarray(MT_OFFSET * spacing + np.minimum(blo, 0.0))
+    hi = np.ascontiguousarray(bhi + 6.0 * spacing)
+    return _mesh(N.gen().pvgen_tree, len(seg), N.dptr(seg), N.dptr(blo), N.dptr(bhi), N.dptr(lo),
+                 N.dptr(hi), spacing)
+
+
 @dataclass
 class Case:
     """A workload ready for SubDomain/Simulation: geometry, faces, inlet profile, tau, notes."""
@@ -103,6 +116,39 @@ def ybif_case(q=27, box=(512, 256, 256), tau=0.6, rp=24.0, rd=17.0, angle=30.0, 
     geo = Geometry.from_triangles(tri, LatticeModel(q), spacing=1.0, grid=((0.0, 0.0, 0.0), box))
     return Case("ybif", geo, xfaces(geo), tau, (ny / 2.0, nz / 2.0 - 0.5), rp,
                 extra={"xb": xb, "rd": rd, "angle": angle})
+
+
+def branch_case(q=19, box=(96, 72, 64), tau=0.8):
+    """A vessel with the reference's outlet arrangement (PV.stl: an x-face inlet, outlet 1 on a
+    +y face with the y-max closure, outlets 2 and 3 on -z faces with the z-min closure,
+    ComplexGeometry.cpp:1376-1408, Simulation.cpp:642-836).  A parent tube (R = 8) enters through
+    the x-min face and ends in a rounded tip; branch 1 (R = 6) leaves through the y-max face of the
+    box, branch 2 (R = 6) through its z-min face, and branch 3 (R = 5) ends in a flat cap on the plane
+    z = 12 inside the box, as PV.stl's outlet 2 does at z = -110 above its outlet 3.  Faces carry
+    finite windows around each opening, and branch 2's opening (nodes x = 45..55 on z = 0) is
+    split on purpose: outlet 2's window x in (46, 53) is open, so x = 46 and x = 53 on its bounds
+    are excluded; a second outlet-3 face on z = 0 with x in (49.5, 60) overlaps it at x = 50..52,
+    where outlet 2 must win (role precedence), and takes x = 53..55; x = 45, 46 match nothing and
+    are wall nodes."""
+    nx, ny, nz = box
+    y0, z0 = 24.0, 40.0
+    segs = [
+        ((-10.0, y0, z0), (70.0, y0, z0), 8.0, 8.0, -1, 0.0, 1),
+        ((30.0, y0, z0), (30.0, ny + 10.0, z0), 6.0, 6.0, -1, 0.0, 1),
+        ((50.0, y0, z0), (50.0, y0, -10.0), 6.0, 6.0, -1, 0.0, 1),
+        ((70.0, y0, z0), (70.0, y0, 12.0), 5.0, 5.0, 2, 12.0, 1),
+    ]
+    rows = [list(a) + list(b) + [ra, rb, ax, v, keep] for a, b, ra, rb, ax, v, keep in segs]
+    tri = tree_mesh(rows, (-2.5, -2.5, -2.5), (nx + 1.5, ny + 1.5, nz + 1.5))
+    geo = Geometry.from_triangles(tri, LatticeModel(q), spacing=1.0, grid=((0.0, 0.0, 0.0), box))
+    faces = [
+        face(N.INLET, N.INLET_XMIN, 0, 0.0, 1e-2, (y0 - 10, z0 - 10), (y0 + 10, z0 + 10)),
+        face(N.OUTLET1, N.OUTLET_YMAX, 1, float(ny - 1), 1e-2, (30.0 - 8, z0 - 8), (30.0 + 8, z0 + 8)),
+        face(N.OUTLET2, N.OUTLET_ZMIN, 2, 0.0, 1e-2, (46.0, y0 - 8), (53.0, y0 + 8)),
+        face(N.OUTLET3, N.OUTLET_ZMIN, 2, 0.0, 1e-2, (49.5, y0 - 8), (60.0, y0 + 8)),
+        face(N.OUTLET3, N.OUTLET_ZMIN, 2, 12.0, 1e-2, (70.0 - 7, y0 - 7), (70.0 + 7, y0 + 7)),
+    ]
+    return Case("branch", geo, faces, tau, (y0, z0 - 0.5), 8.0)
 
 
 def vessel_case(seed, q=19, n=96, tau=0.7):

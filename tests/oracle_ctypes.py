@@ -61,6 +61,8 @@ def load():
         "orc_sim_gate_events": (C.c_int64, [C.c_void_p]),
         "orc_collide": (None, [C.c_int, C.c_double, _d, _d, _d, _d]),
         "orc_zouhe": (None, [C.c_int, C.c_int, _d, C.c_double, C.c_double]),
+        "orc_wss_node": (None, [C.c_int, _d, _d, _d, C.c_double, _d]),
+        "orc_classify_point": (C.c_int, [C.c_void_p, C.c_int, _d]),
         "orc_pulsatile_velocity": (C.c_double, [C.c_double, C.c_double]),
         "orc_lattice": (None, [C.c_int, _i32, _d, _i32]),
     }
@@ -149,6 +151,22 @@ def zouhe(q, closure, f, ux=0.0, rho=1.0):
     g = np.array(f, dtype=np.float64).copy()
     load().orc_zouhe(q, closure, _dp(g), ux, rho)
     return g
+
+
+def wss_node(q, f, feq, n, C):
+    tau = np.zeros(3)
+    load().orc_wss_node(q, _dp(np.ascontiguousarray(f, dtype=np.float64)),
+                        _dp(np.ascontiguousarray(feq, dtype=np.float64)),
+                        _dp(np.ascontiguousarray(n, dtype=np.float64)), float(C), _dp(tau))
+    return tau
+
+
+def classify_point(faces, c):
+    """Index of the face a boundary point at c is assigned to, or -1 (wall)."""
+    from pv_dec_original_amd import _native as N
+    arr = (N.Face * max(1, len(faces)))(*faces)
+    return load().orc_classify_point(C.cast(arr, C.c_void_p), len(faces),
+                                     _dp(np.ascontiguousarray(c, dtype=np.float64)))
 
 
 class OracleSim:

@@ -117,3 +117,18 @@ def test_missing_library_fails_loudly(tmp_path, monkeypatch):
         monkeypatch.setattr(_native, name, None)
         with pytest.raises(_native.NativeError, match="missing"):
             loader()
+
+
+def test_grid_beyond_the_tile_index_range_is_rejected_before_any_device_call():
+    """The step addresses planes with 64-bit bases, so a plane may exceed 4 GiB; what remains
+    bounded is the 32-bit index of a 64-node tile.  A grid with more than 2^31 tiles is rejected
+    with PVLBM_EINVAL before any HIP call (here, on a machine without a GPU, a HIP call would fail
+    with PVLBM_EHIP instead)."""
+    g = N.Grid()
+    g.dims[0], g.dims[1], g.dims[2] = 1 << 16, 1 << 15, 1 << 12
+    fluid = np.zeros(8, dtype=np.uint8)  # never read: the size check comes first
+    h = C.c_void_p()
+    rc = N.lib().pvlbm_create(N.u8ptr(fluid), C.byref(g), 1.0, 19, N.FP64, None, 0, None, 0, 0.8, 8e-6,
+                              C.byref(h))
+    assert rc == 1 and not h.value
+    assert b"too large" in N.lib().pvlbm_last_error(None)

@@ -1,0 +1,90 @@
+"""Population planes larger than 4 GiB.
+
+The step addresses each direction plane from a 64-bit wave-uniform base plus a lane offset, so a
+plane may hold more than 2^32 bytes.  The tube's voxel flags are placed at the top of an
+81 x 21 x 210,000 box, whose FP64 planes are 4.7 GB each (about 200 GB of device memory for the two
+D3Q19 lattices), so every population of the tube lives beyond byte 2^32 of its plane.  The run must
+reproduce the same tube in its own 81 x 21 x 20 box bit for bit (density and velocity of every
+node), which a wrapped 32-bit offset would not: it would read and write other slots, or fault.
+"""
+import numpy as np
+import pytest
+
+from pv_dec_original_amd import Simulation, SubDomain
+from pv_dec_original_amd import _native as N
+from pv_dec_original_amd.geometry import Geometry
+from pv_dec_original_amd.lattice import LatticeModel
+from pv_dec_original_amd.workloads import tube_case, xfaces
+
+pytestmark = pytest.mark.gpu
+
+NZ_LARGE = 210_000
+
+
+def _geometry(fluid, origin, tri):
+    g = Geometry.__new__(Geometry)
+    g.m_velSet = LatticeModel(19)
+    g.spacing = 1.0
+    g.raydirection = "z"
+    g.globalTriangles = tri
+    g.boundingbox = None
+    g.fluid = np.ascontiguousarray(fluid)
+    g.origin = np.asarray(origin, dtype=np.float64)
+    g.nz, g.ny, g.nx = fluid.shape
+    g.stlFileName = None
+    return g
+
+
+def _run(geo, faces, scale, prof):
+    sd = SubDomain(geo, faces, storage="fp64", tau=0.8, dt=8e-6, triangles=False)
+    sim = Simulation(sd, 0.8, 1.0, 8e-6)
+    sim.set_inlet_profile(prof)
+    sim.step(len(scale), scale)
+    rho, u = sim.macros_device()
+    red = sim.reduce()
+    lists = {lid: np.sort(sd.list(lid)) for lid in (N.LIST_FLUID, N.LIST_INLET, N.LIST_OUTLET1, N.LIST_WALL)}
+    return sd, rho, u, red, lists
+
+
+def test_planes_beyond_4_gib_match_a_small_box():
+    import torch
+    free, _ = torch.cuda.mem_get_info()
+    if free < 215e9:
+        pytest.skip(f"needs about 200 GB of free device memory, {free / 1e9:.0f} GB free")
+    case = tube_case(q=19)
+    small = case.geometry
+    nz, ny, nx = small.fluid.shape
+    nxp, nyp = (nx + 1 + 63) // 64 * 64, ny + 1
+    plane_bytes = 8 * (nxp * nyp * NZ_LARGE + 2 * (nxp * nyp + nxp + 64))
+    assert plane_bytes > (1 << 32) + (1 << 28)
+    # the tube at the top of the tall box; node coordinates are not needed to match (no triangles:
+    # normals and WSS are off), only the kinds, which the x faces fix
+    big = np.zeros((NZ_LARGE, ny, nx), dtype=np.uint8)
+    big[NZ_LARGE - nz:] = small.fluid
+    geo_s = _geometry(small.fluid, small.origin, np.zeros((0, 12)))
+    geo_b = _geometry(big, (small.origin[0], small.origin[1], small.origin[2] - (NZ_LARGE - nz)), np.zeros((0, 12)))
+    faces = xfaces(small)
+    rng = np.random.default_rng(4)
+    scale = rng.uniform(0.6, 1.0, 150)
+    sd_s = SubDomain(geo_s, faces, storage="fp64", tau=0.8, dt=8e-6, triangles=False)
+    prof = case.inlet_profile(sd_s, 0.05)
+    del sd_s
+    print(f"large planes: box {nx}x{ny}x{NZ_LARGE}, {plane_bytes / 1e9:.2f} GB per plane", flush=True)
+    sd_s, rho_s, u_s, red_s, lists_s = _run(geo_s, faces, scale, prof)
+    kinds_s = sd_s.kinds()
+    del sd_s
+    sd_b, rho_b, u_b, red_b, lists_b = _run(geo_b, faces, scale, prof)
+    del sd_b
+    shift = nx * ny * (NZ_LARGE - nz)  # box index of the tube's first plane in the tall box
+    for lid, members in lists_s.items():
+        assert np.array_equal(lists_b[lid] - shift, members), lid
+    top = slice(NZ_LARGE - nz, NZ_LARGE)
+    # the device fields are FP32 views of the FP64 state: equal FP64 states give equal bits
+    assert torch.equal(rho_b[top].cpu(), rho_s.cpu())
+    assert torch.equal(u_b[:, top].cpu(), u_s.cpu())
+    k = torch.from_numpy(kinds_s != N.SOLID)
+    assert float(rho_s.cpu()[k].min()) > 0.9 and float(u_s.cpu()[0][k].abs().max()) > 0.01
+    # global FP64 sums: same nodes, another block grouping, so equal to rounding
+    assert abs(red_b[0] - red_s[0]) <= 1e-12 * red_s[0]
+    assert abs(red_b[1] - red_s[1]) <= 1e-12 * abs(red_s[1])
+    assert red_b[3] == 0 and red_s[3] == 0

@@ -1,0 +1,94 @@
+"""The reference's own outlet closures and windowed face classification on the device.
+
+PV.stl's outlets are a +y face with the y-max Zou-He pressure closure (Simulation.cpp:642-695) and
+two -z faces with the z-min closure (:710-836), picked out by long double planes with finite
+windows (ComplexGeometry.cpp:1376-1408) in the precedence inlet, outlet 1, 2, 3
+(Subdomain.cpp:124-155).  workloads.branch_case builds that arrangement on a generated vessel,
+with windows that cut an opening in two and overlap (see its docstring).  Voxel flags, node kinds
+and every list must equal the oracle's; rho, u, WSS, its vector and the accumulators must agree
+bit for bit over a pulsatile run.
+"""
+import numpy as np
+import pytest
+
+from pv_dec_original_amd import _native as N
+from pv_dec_original_amd.simulation import face
+from tests import gpu_cases as G
+from tests import oracle_ctypes as O
+from tests.test_gpu_parity import _oracle_labels
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def test_branch_outlets_ymax_zmin_bit_exact():
+    from pv_dec_original_amd.workloads import branch_case
+    case = branch_case(q=19)
+    geo = case.geometry
+    assert np.array_equal(geo.fluid, _oracle_labels(geo.globalTriangles, (0.0, 0.0, 0.0), geo.dims,
+                                                    binned=True))
+    sd, sim, orc = G.build(case, u_max=0.05)
+    k = sd.kinds()
+    assert np.array_equal(k, orc.kinds())
+    for lid in (N.LIST_FLUID, N.LIST_INLET, N.LIST_OUTLET1, N.LIST_OUTLET2, N.LIST_OUTLET3,
+                N.LIST_WALL, N.LIST_NTW):
+        assert np.array_equal(np.sort(sd.list(lid)), np.sort(orc.list(lid))), lid
+    nx, ny = geo.nx, geo.ny
+    # the classification the faces were built for: outlet 2 only strictly inside x in (46, 53),
+    # outlet 3 on both of its planes, the rim columns x = 45, 46 of branch 2 on z = 0 walls
+    o2, o3 = sd.list(N.LIST_OUTLET2), sd.list(N.LIST_OUTLET3)
+    assert sorted(set((o2 % nx).tolist())) == [47, 48, 49, 50, 51, 52]
+    assert sorted(set((o2 // (nx * ny)).tolist())) == [0]
+    assert sorted(set((o3 // (nx * ny)).tolist())) == [0, 12]
+    assert set((o3[o3 // (nx * ny) == 0] % nx).tolist()) == {53, 54, 55}
+    assert k[0, 24, 45] == N.WALL and k[0, 24, 46] == N.WALL
+    assert len(sd.list(N.LIST_OUTLET1)) > 50 and len(sd.list(N.LIST_INLET)) > 100
+    assert set((sd.list(N.LIST_OUTLET1) // nx % ny).tolist()) == {ny - 1}
+
+    steps = 240
+    t = np.arange(steps)
+    scale = 0.8 + 0.2 * np.sin(2 * np.pi * t / 160.0)
+    for chunk in (steps // 2, steps - steps // 2):
+        done = sim.iteration
+        sim.step(chunk, scale[done:done + chunk])
+        orc.step(chunk, scale[done:done + chunk], wss_every=1)
+        rho, u = sim.macros()
+        rho_o, u_o = orc.macros()
+        assert G.rel_err(rho, rho_o) <= TOL
+        assert G.rel_err(u, u_o) <= TOL
+        w = sim.wss()
+        wo, tau_o, acc_o = orc.wss()
+        assert G.rel_err(w[:, 3], wo[:, 3]) <= TOL and G.rel_err(w[:, 5], wo[:, 5]) <= TOL
+        tau_g, acc_g = sim.wss_raw()
+        assert G.rel_err(tau_g, tau_o) <= TOL and G.rel_err(acc_g, acc_o) <= TOL
+    # the closures did their job: prescribed density on the pressure outlets, flow through them
+    out1 = sd.list(N.LIST_OUTLET1)
+    assert np.abs(rho.reshape(-1)[out1] - 1.0).max() < 0.05
+    vy = u[1].reshape(-1)[out1]
+    vz = u[2].reshape(-1)[sd.list(N.LIST_OUTLET2)]
+    assert vy.mean() > 0 and vz.mean() < 0  # leaving through +y and through -z
+    assert sim.gate_events() == 0 and orc.gate_events() == 0
+    assert np.isfinite(rho).all() and np.isfinite(u).all()
+
+
+def test_face_plane_low_part_on_device():
+    """Face planes that no double equals: the tube's inlet plane placed 0.01 + 1e-19 and
+    0.01 + 3e-19 below x = 0, with the tolerance 1e-2 (whose double is 0.01 + 2.1e-19).  The nodes
+    on x = 0 match the first and not the second; both planes round to the double -0.01, which they
+    would match, so the engine (like the oracle) must compare against the long double."""
+    from pv_dec_original_amd.workloads import tube_case
+    base = tube_case(q=19)
+    geo = base.geometry
+    x1 = geo.origin[0] + (geo.nx - 1) * geo.spacing
+    inside = "-0.0100000000000000001"     # 1e-19 beyond 0.01, within the tolerance
+    outside = "-0.0100000000000000003"    # 3e-19 beyond 0.01, outside the tolerance
+    for plane, expect_inlet in ((inside, True), (outside, False)):
+        f_in = face(N.INLET, N.INLET_XMIN, 0, plane, 1e-2)
+        assert float(np.longdouble(plane)) == -0.01  # the double alone would match
+        faces = [f_in, face(N.OUTLET1, N.OUTLET_XMAX, 0, x1, 1e-2)]
+        assert (O.classify_point(faces, (0.0, 0.0, 0.0)) == 0) == expect_inlet
+        base.faces = faces
+        sd, sim, orc = G.build(base, normals=False)
+        assert np.array_equal(sd.kinds(), orc.kinds())
+        assert (sd.count(N.LIST_INLET) > 0) == expect_inlet
