@@ -181,6 +181,18 @@ int pvlbm_get_kinds(pvlbm_handle h, uint8_t* kinds);
 int pvlbm_sample_kinds(pvlbm_handle h, const int32_t* ix, int32_t nx, const int32_t* iy, int32_t ny,
                        const int32_t* iz, int32_t nz, uint8_t* out);
 
+/* As pvlbm_sample_kinds, allocating nothing.  No counterpart in the reference: it serves callers
+ * that cannot allocate device memory (serving loops that sample many handles, harnesses that
+ * forbid hipMalloc).  The sample indices and the sampled kinds are staged in a device array of
+ * PVLBM_NOALLOC_SCRATCH_BYTES (8 MiB) that every GPU the library runs on holds from the first
+ * pvlbm_create on it until the process ends; a request needing more than that, 4 * (nx + ny + nz)
+ * bytes rounded up to 256 plus nx * ny * nz bytes, fails with PVLBM_EINVAL.  All calls on one GPU
+ * share that array (each writes every byte it reads back), so calls that overlap in time on one
+ * GPU race. */
+#define PVLBM_NOALLOC_SCRATCH_BYTES (8 << 20)
+int pvlbm_sample_kinds_noalloc(pvlbm_handle h, const int32_t* ix, int32_t nx, const int32_t* iy, int32_t ny,
+                               const int32_t* iz, int32_t nz, uint8_t* out);
+
 /* Inlet velocity of each inlet-list node; the x-velocity applied at step s is
  * inlet_scale[s] * u[node].  Default 1.0 everywhere, which makes inlet_scale the reference's
  * uniform InletVelocity_ (Simulation.cpp:580, Boundary.cpp:131-133). */

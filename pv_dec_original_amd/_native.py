@@ -93,6 +93,8 @@ PVLBM_API = [
     ("pvlbm_get_kinds", C.c_int, [C.c_void_p, _u8]),
     ("pvlbm_sample_kinds", C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.c_int32, C.POINTER(C.c_int32), C.c_int32,
                                      C.POINTER(C.c_int32), C.c_int32, _u8]),
+    ("pvlbm_sample_kinds_noalloc", C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.c_int32, C.POINTER(C.c_int32), C.c_int32,
+                                     C.POINTER(C.c_int32), C.c_int32, _u8]),
     ("pvlbm_set_inlet_profile", C.c_int, [C.c_void_p, _d]),
     ("pvlbm_init_fields", C.c_int, [C.c_void_p, _d, _d]),
     ("pvlbm_init_fields_coarse", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_int32),

@@ -25,7 +25,7 @@ HIP_FLAGS = [
 ]
 
 LIBS = {
-    "libpvlbm.so": ["capi.cpp", "k_step.hip", "k_setup.hip"],
+    "libpvlbm.so": ["capi.cpp", "k_step.hip", "k_setup.hip", "noalloc.hip"],
     "libpvnet.so": ["k_net.hip"],
 }
 

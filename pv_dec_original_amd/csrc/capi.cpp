@@ -102,6 +102,8 @@ struct pvlbm_sim {
     std::string err;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int device = 0;            // the device the handle's memory lives on
+    void* scratch = nullptr;   // that device's scratch array (noalloc.hip), never freed
     int q = 19, storage = PVLBM_FP64;
     // padded box: nxp >= nx + 1 (multiple of 64) and nyp = ny + 1, so that every out-of-box
     // neighbour position of a node is a non-node slot (the home of a stale population)
@@ -561,6 +563,8 @@ extern "C" int pvlbm_create(const uint8_t* fluid, const pvlbm_grid* grid, double
             double dxdx = h * h * 1e-6;
             S->C = phys_den * dtdt / dxdx;
         }
+        HIPCHK(hipGetDevice(&S->device));
+        HIPCHK(noalloc_scratch(S->device, &S->scratch));
         HIPCHK(hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking));
         HIPCHK(hipEventCreate(&S->ev0));
         HIPCHK(hipEventCreate(&S->ev1));
@@ -829,18 +833,28 @@ extern "C" int pvlbm_get_kinds(pvlbm_handle h, uint8_t* kinds) {
     });
 }
 
+namespace {
+// Checks a separable sample grid against the box and returns its point count; the product is
+// bounded before it is formed.
+size_t check_sample_grid(const pvlbm_sim* h, const int32_t* ix, int32_t nx, const int32_t* iy, int32_t ny,
+                         const int32_t* iz, int32_t nz, const uint8_t* out) {
+    if (!ix || !iy || !iz || !out || nx < 1 || ny < 1 || nz < 1) fail(PVLBM_EINVAL, "bad sample grid");
+    if ((double)nx * (double)ny * (double)nz > 4.0e18) fail(PVLBM_EINVAL, "sample grid too large");
+    const int32_t* idx[3] = {ix, iy, iz};
+    const int32_t cnt[3] = {nx, ny, nz};
+    const int64_t dims[3] = {h->nx, h->ny, h->nz};
+    for (int a = 0; a < 3; a++)
+        for (int32_t i = 0; i < cnt[a]; i++)
+            if (idx[a][i] < 0 || idx[a][i] >= dims[a]) fail(PVLBM_EINVAL, "sample index outside the box");
+    return (size_t)nx * (size_t)ny * (size_t)nz;
+}
+}  // namespace
+
 extern "C" int pvlbm_sample_kinds(pvlbm_handle h, const int32_t* ix, int32_t nx, const int32_t* iy, int32_t ny,
                                   const int32_t* iz, int32_t nz, uint8_t* out) {
     if (!h) return PVLBM_EINVAL;
     return guarded(h, [&] {
-        if (!ix || !iy || !iz || !out || nx < 1 || ny < 1 || nz < 1) fail(PVLBM_EINVAL, "bad sample grid");
-        const int32_t* idx[3] = {ix, iy, iz};
-        const int32_t cnt[3] = {nx, ny, nz};
-        const int64_t dims[3] = {h->nx, h->ny, h->nz};
-        for (int a = 0; a < 3; a++)
-            for (int32_t i = 0; i < cnt[a]; i++)
-                if (idx[a][i] < 0 || idx[a][i] >= dims[a]) fail(PVLBM_EINVAL, "sample index outside the box");
-        const size_t n = (size_t)nx * ny * nz;
+        const size_t n = check_sample_grid(h, ix, nx, iy, ny, iz, nz, out);
         int32_t* d_idx = nullptr;
         uint8_t* d_out = nullptr;
         try {
@@ -861,6 +875,31 @@ extern "C" int pvlbm_sample_kinds(pvlbm_handle h, const int32_t* ix, int32_t nx,
         }
         dfree(d_idx);
         dfree(d_out);
+    });
+}
+
+extern "C" int pvlbm_sample_kinds_noalloc(pvlbm_handle h, const int32_t* ix, int32_t nx, const int32_t* iy,
+                                          int32_t ny, const int32_t* iz, int32_t nz, uint8_t* out) {
+    if (!h) return PVLBM_EINVAL;
+    return guarded(h, [&] {
+        const size_t n = check_sample_grid(h, ix, nx, iy, ny, iz, nz, out);
+        // [indices, padded to 256 bytes][sampled kinds] in the scratch array; every byte that is
+        // read back is written by this call, so nothing carried from an earlier call is seen
+        const size_t idx_bytes = ((size_t)nx + ny + nz) * sizeof(int32_t);
+        const size_t out_off = (idx_bytes + 255) / 256 * 256;
+        if (n > (size_t)PVLBM_NOALLOC_SCRATCH_BYTES || out_off + n > (size_t)PVLBM_NOALLOC_SCRATCH_BYTES)
+            fail(PVLBM_EINVAL, "pvlbm_sample_kinds_noalloc: %zu samples and %zu index bytes exceed the %d-byte "
+                 "scratch array", n, idx_bytes, (int)PVLBM_NOALLOC_SCRATCH_BYTES);
+        if (!h->scratch) fail(PVLBM_ESTATE, "no scratch array on this handle's device");
+        int32_t* d_idx = static_cast<int32_t*>(h->scratch);
+        uint8_t* d_out = static_cast<uint8_t*>(h->scratch) + out_off;
+        HIPCHK(hipMemcpyAsync(d_idx, ix, sizeof(int32_t) * nx, hipMemcpyHostToDevice, h->stream));
+        HIPCHK(hipMemcpyAsync(d_idx + nx, iy, sizeof(int32_t) * ny, hipMemcpyHostToDevice, h->stream));
+        HIPCHK(hipMemcpyAsync(d_idx + nx + ny, iz, sizeof(int32_t) * nz, hipMemcpyHostToDevice, h->stream));
+        HIPCHK(launch_sample_kinds(h->d_kind, h->nxp, h->nyp, d_idx, nx, d_idx + nx, ny, d_idx + nx + ny, nz, d_out,
+                                   h->stream));
+        HIPCHK(hipMemcpyAsync(out, d_out, n, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
     });
 }
 

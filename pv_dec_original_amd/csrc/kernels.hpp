@@ -112,6 +112,10 @@ hipError_t launch_link_q(const double* seg, int64_t npairs, const double* tri, c
                          const int32_t* cell_tri, const double og[3], double cs, const int32_t nc[3], double h,
                          int nearest, double* q, hipStream_t s);
 
+// Device address of `device`'s scratch array of PVLBM_NOALLOC_SCRATCH_BYTES (noalloc.hip): looked
+// up once per device with that device current, then cached.
+hipError_t noalloc_scratch(int device, void** ptr);
+
 // streaming copy of `bytes` (multiple of 16) for counter calibration
 hipError_t launch_copy(const void* src, void* dst, int64_t bytes, hipStream_t s);
 
