@@ -8,9 +8,14 @@ inlet periods.
 
 Output: one JSON line.  value = fluid-node updates per second / 1e6 (MLUPS) over the timed steps,
 inputs resident in HBM.  roofline.achieved = fluid nodes * 2 * Q * 8 bytes / average launch time.
+reference_cadence = the same handle timed with WSS and its OSI/TAWSS accumulators evaluated every
+iteration, as the reference does (Simulation.cpp:585-586); the workload's own cadence (WSS every
+100 iterations) is the headline.  roofline.copy_kernel_GBs = a 4-deep 16-byte nontemporal copy,
+the attainable HBM rate of a plain stream.
 cpu_baseline = the CPU oracle (a restatement of the reference solver, push streaming) on a bounded
 number of steps of the same workload, single-threaded like one reference MPI rank;
-cpu_baseline_threads = the same over the host cores the process may use (OpenMP).
+cpu_baseline_threads = the same over every host core the process may use (OpenMP; capped by an
+OMP_NUM_THREADS the environment sets as the host's CPU share).
 --workload box / aneurysm run the all-fluid box (the kernel's ceiling) or the aneurysm instead.
 
 "learned" (on by default, --learned 0 to skip): wall time to converged wall shear stress on the
@@ -98,6 +103,9 @@ def main():
                     help="ybif: the Y-bifurcation; box: an all-fluid box (kernel ceiling); "
                          "aneurysm: the curved tube with a saccular bulge of the learned-start case")
     ap.add_argument("--wss-every", type=int, default=100)
+    ap.add_argument("--ref-cadence-steps", type=int, default=2000,
+                    help="also time this many steps with WSS every iteration, the reference's cadence "
+                         "(0 = skip)")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of oracle CPU work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-bytes", type=int, default=1 << 30,
@@ -174,6 +182,20 @@ def main():
     bytes_per_flup = 2 * q * (8 if args.storage == "fp64" else 2)
     per_launch_s = sec / K
     achieved = nfl * bytes_per_flup / per_launch_s / 1e9
+
+    # the reference's cadence beside the workload's: WSS and its OSI/TAWSS accumulators every
+    # iteration (Simulation.cpp:585-586), timed on the same handle over the following steps
+    ref_cadence = None
+    if args.wss_every != 1 and args.ref_cadence_steps > 0:
+        K1 = args.ref_cadence_steps
+        sim.set_wss_every(1)
+        ms1 = sim.step(K1, inlet_scales(args.warmup + K, K1, period, mean, amp), timed=True)
+        sim.set_wss_every(args.wss_every)
+        s1 = ms1 / 1e3 / K1
+        ref_cadence = {"wss_every": 1, "steps": K1, "ms_per_step": round(ms1 / K1, 5),
+                       "value": round(nfl / s1 / 1e6, 2), "unit": "MLUPS",
+                       "roofline_frac": round(nfl * bytes_per_flup / s1 / 1e9 / HBM_PEAK_GBS, 4)}
+        log(f"WSS every iteration: {ms1 / K1 * 1e3:.2f} us per step")
     dtype = {"fp64": "f64", "fp16": "f16-storage/f32", "bf16": "bf16-storage/f32"}[args.storage]
 
     copy_gbs = None
@@ -194,12 +216,18 @@ def main():
                          f"-O2 -ffp-contract=off, 1 thread) in {el:.1f} s on {cpu_model()}; "
                          "normals skipped (timing only)"}
         log(f"cpu baseline {v:.2f} MLUPS ({n} steps)")
-        # the same oracle over the host cores this process may use (a second, threaded baseline)
-        nthr = max(1, min(len(os.sched_getaffinity(0)), int(os.environ.get("OMP_NUM_THREADS", "16") or 16)))
+        # the same oracle over every host core this process may use (a second, threaded baseline);
+        # an OMP_NUM_THREADS set by the environment (a host's declared CPU share) caps it
+        allowed = len(os.sched_getaffinity(0))
+        omp = os.environ.get("OMP_NUM_THREADS", "")
+        nthr = max(1, min(allowed, int(omp))) if omp.isdigit() and int(omp) > 0 else allowed
         vt, nt, elt = cpu_baseline(case, pbox, q, tau, args.cpu_budget / 2, nthr)
         cpu_threads = {"value": round(vt, 3), "unit": "MLUPS", "cores": nthr, "kind": "port",
-                       "sample": f"{nt} iterations by the CPU oracle with OpenMP over {nthr} threads in "
-                                 f"{elt:.1f} s on {cpu_model()}; normals skipped (timing only)"}
+                       "cores_allowed": allowed, "cpu_count": os.cpu_count(),
+                       "omp_num_threads": omp or None,
+                       "sample": f"{nt} iterations by the CPU oracle with OpenMP over {nthr} threads "
+                                 f"({allowed} cores allowed to the process, {os.cpu_count()} in the machine) "
+                                 f"in {elt:.1f} s on {cpu_model()}; normals skipped (timing only)"}
         log(f"cpu baseline, {nthr} threads: {vt:.2f} MLUPS ({nt} steps)")
 
     workload = f"{args.workload}-D3Q{q}-{args.storage}-{box[0]}x{box[1]}x{box[2]}"
@@ -221,6 +249,8 @@ def main():
             "lattice": q, "storage": args.storage, "box": list(box), "fluid_nodes": nfl,
             "fluid_fraction": round(nfl / nbox, 4), "tau": tau,
             "inlet": "0.03+0.02*sin(2*pi*t/4000) x parabolic(mean 1)", "wss_every": args.wss_every,
+            "wss_accumulators": ("every iteration" if args.wss_every == 1 else
+                                 f"sampled every {args.wss_every} iterations (OSI/TAWSS sums over the samples)"),
             "lists": counts, "population_stores": "nontemporal" if sim.streaming_stores() else "plain",
             "wall_s_timed": round(wall, 3), "setup_s": round(t_setup, 2),
             "voxelise_s": round(t_geo, 2),
@@ -237,6 +267,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "cpu_baseline_threads": cpu_threads,
+        "reference_cadence": ref_cadence,
     }
     if args.learned:
         del sd, sim  # free the Y-bifurcation's device memory first

@@ -1,5 +1,6 @@
 """Wall time to converged WSS on the large aneurysm: plain FP64 from rest against FP16 shifted
-storage started from the learned warm start (and the two ablations).
+storage started from the learned warm start, the same start without the network (plug flow with
+the same plane fluxes and phase-0 profiles: the network's own share), and the ablations.
 
 Workload: a seeded curved tube of radius 40 with a spherical saccular bulge of radius 70 in a
 1024 x 512 x 512 box (workloads.aneurysm_case), D3Q19, tau = 0.6, parabolic inlet of unit mean
@@ -52,6 +53,8 @@ def run(weights=DEFAULT_WEIGHTS, box=(1024, 512, 512), tau=0.6, period=4000, sam
     warm = W.WarmStart(weights)
     warm.prepare(box[::-1])
     net_setup_s = time.perf_counter() - t
+    plug = W.PlugStart(warm.net.spec.divisor)
+    plug.prepare(box[::-1])
 
     flux = {}
 
@@ -80,8 +83,9 @@ def run(weights=DEFAULT_WEIGHTS, box=(1024, 512, 512), tau=0.6, period=4000, sam
     def rest(sim):
         return lambda: (sim.init_fields(), "rest")[1]
 
-    def learned(sim, sd, phase0=True):
-        return lambda: W.start(sim, sd, warm, mean, tau, flux["q"], (amp, period) if phase0 else None)
+    def learned(sim, sd, phase0=True, source=None):
+        src = warm if source is None else source
+        return lambda: W.start(sim, sd, src, mean, tau, flux["q"], (amp, period) if phase0 else None)
 
     # the plain path, continued past its own convergence to a tighter reference record
     for lab in ("fp64-rest", "fp64-warm"):
@@ -97,10 +101,13 @@ def run(weights=DEFAULT_WEIGHTS, box=(1024, 512, 512), tau=0.6, period=4000, sam
                                    stop_change=tol, reference=ref, start=learned(sim64, sd64), log=log))
     del sim64, sd64
     sd16, sim16 = make("fp16")
-    for lab in ("fp16-warm", "fp16-warm-mean", "fp16-rest"):
+    for lab in ("fp16-warm", "fp16-plug", "fp16-warm-mean", "fp16-rest"):
         stores[lab] = "nontemporal" if sim16.streaming_stores() else "plain"
     runs.append(CV.run_periods(sim16, "fp16-warm", period, sample_every, mean, amp, max_periods,
                                stop_change=tol, reference=ref, start=learned(sim16, sd16), log=log))
+    # the same start without the network: plug flow with the same plane fluxes and phase-0 profiles
+    runs.append(CV.run_periods(sim16, "fp16-plug", period, sample_every, mean, amp, max_periods,
+                               stop_change=tol, reference=ref, start=learned(sim16, sd16, source=plug), log=log))
     if ablations:
         runs.append(CV.run_periods(sim16, "fp16-warm-mean", period, sample_every, mean, amp, max_periods,
                                    stop_change=tol, reference=ref, start=learned(sim16, sd16, False), log=log))
@@ -122,7 +129,11 @@ def run(weights=DEFAULT_WEIGHTS, box=(1024, 512, 512), tau=0.6, period=4000, sam
     def ratio(a, b):
         return round(a / b, 3) if a and b else None
 
+    g = out["runs"]["fp16-plug"]
     out["speedup_wall"] = ratio(p["converged_wall_s"], c["converged_wall_s"])
+    # the network's own share: the same path started from plug flow instead of the network's field
+    out["speedup_wall_plug_start"] = ratio(p["converged_wall_s"], g["converged_wall_s"])
+    out["network_gain_wall"] = ratio(g["converged_wall_s"], c["converged_wall_s"])
     out["speedup_wall_to_reference"] = ratio(p["within_tol_of_reference_wall_s"], c["within_tol_of_reference_wall_s"])
     out["chosen_error_vs_fp64"] = c["error_at_convergence"]
     return out

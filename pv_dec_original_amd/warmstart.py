@@ -254,12 +254,15 @@ class WarmStart:
         NET_MAX voxels per axis, sizes divisible by the network's divisor), else a grid with the
         longest axis at NET_MAX and the others scaled alike, each rounded to the divisor.
         Returns (coarse shape, scale s = fine / coarse along the longest axis)."""
-        div = self.net.spec.divisor
+        div = self._divisor()
         if max(shape) <= NET_MAX and all(n % div == 0 for n in shape):
             return tuple(shape), 1.0
         s = max(shape) / float(NET_MAX // div * div)
         s = max(s, 1.0)
         return tuple(max(div, int(round(n / s / div)) * div) for n in shape), s
+
+    def _divisor(self):
+        return self.net.spec.divisor
 
     def predict(self, code, u_mean, tau):
         """(rho, u) on the grid of `code` (d, h, w) as FP32 device tensors (d, h, w) and
@@ -350,6 +353,33 @@ class WarmStart:
                 q = 0.0
         nonfin = sim.init_fields_coarse(rho, u, UMAX, RHO_CLIP, q, targets, drho)
         return nonfin == 0
+
+
+class PlugStart(WarmStart):
+    """The learned start without the network, as its baseline: plug flow, u = (u_mean, 0, 0) and
+    rho = 1 on every fluid voxel of the same coarse grid, then everything else WarmStart.apply does
+    (each plane scaled to its flux, the phase-0 plane fluxes and densities of the pulsatile model,
+    the device expansion).  What a start from WarmStart gains over this one is the network's."""
+
+    def __init__(self, divisor=8):
+        self.net = None
+        self.divisor = divisor
+
+    def _divisor(self):
+        return self.divisor
+
+    def prepare(self, shape):
+        from . import phase0  # noqa: F401  (scipy.special), as WarmStart.prepare loads it
+
+    def coarse_fields(self, kinds, u_mean, tau, sd=None):
+        import torch
+        cshape, _ = self.net_grid(kinds.shape if kinds is not None else (sd.nz, sd.ny, sd.nx))
+        cc = coarse_code(kinds, cshape) if kinds is not None else coarse_code_subdomain(sd, cshape)
+        fl = torch.from_numpy(cc > 0).to("cuda")
+        rho = torch.ones(cshape, dtype=torch.float32, device="cuda")
+        u = torch.zeros((3,) + tuple(cshape), dtype=torch.float32, device="cuda")
+        u[0] = torch.where(fl, torch.full_like(u[0], float(u_mean)), u[0])
+        return rho, u, cc
 
 
 def field_ok(rho, u, fluid):
