@@ -206,7 +206,9 @@ int pvlbm_set_inlet_profile(pvlbm_handle h, const double* u);
  * converges to the same steady state as a start from rest. */
 int pvlbm_init_fields(pvlbm_handle h, const double* rho, const double* u);
 
-/* As pvlbm_init_fields from a coarse FP32 field in device memory: rho_c[cz][cy][cx] and
+/* As pvlbm_init_fields from a coarse FP32 field in device memory (on this handle's device; the call
+ * first waits for all work on the device, so fields still being produced on any stream of the
+ * caller are complete before they are read): rho_c[cz][cy][cx] and
  * u_c[3][cz][cy][cx] (x fastest; cdims = {cx, cy, cz}) cover the grid, coarse voxel (i, j, k)
  * centred at fine coordinates ((i + 0.5) nx / cx - 0.5, ...); each node takes the trilinear
  * sample (clamped at the edges) with its density clipped into [rho_lo, rho_hi].  With
@@ -260,7 +262,10 @@ int pvlbm_set_store_policy(pvlbm_handle h, int policy);
 int pvlbm_get_store_policy(pvlbm_handle h, int* streaming);
 
 /* Every how many iterations WSS and its accumulators are evaluated (reference: every iteration,
- * Simulation.cpp:585-586).  Default 1. */
+ * Simulation.cpp:585-586).  Default 1, the reference's quantities.  With every > 1 the accumulators
+ * sum only the evaluated iterations (index multiple of every), so the OSI and TAWSS that
+ * pvlbm_get_wss reports are those of the sampled sequence (TAWSS = sum over the samples of |tau|
+ * / 0.9), not the reference's every-iteration sums. */
 int pvlbm_set_wss_every(pvlbm_handle h, int every);
 
 /* Run n iterations of Simulation::Iterate's body (Simulation.cpp:577-586: collide, stream, inlet,

@@ -957,6 +957,9 @@ extern "C" int pvlbm_init_fields_coarse(pvlbm_handle h, const float* rho_c, cons
         for (int a = 0; a < 3; a++)
             if (cdims[a] < 1) fail(PVLBM_EINVAL, "coarse dims must be positive");
         if (!(umax > 0.0) || !(rho_lo <= rho_hi)) fail(PVLBM_EINVAL, "bad clipping limits");
+        // the coarse fields are the caller's device memory, possibly still being written on the
+        // caller's stream: wait for all work on the device before this handle's stream reads them
+        HIPCHK(hipDeviceSynchronize());
         double* d_in = nullptr;
         unsigned long long* d_cnt = nullptr;
         try {

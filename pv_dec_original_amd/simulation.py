@@ -383,7 +383,10 @@ class Simulation:
         return path
 
     def WriteOutput(self, iteration, outdir):
-        """FlowHemodynamics/<iter>_FlowHemodynamics.csv (OutputFile.cpp:867-938)."""
+        """FlowHemodynamics/<iter>_FlowHemodynamics.csv (OutputFile.cpp:867-938): the last WSS
+        evaluation and the OSI and TAWSS of the accumulators.  These are the reference's
+        quantities with WSS evaluated every iteration (set_wss_every(1), the default); with a
+        coarser cadence the accumulators hold the sampled iterations only (pvlbm_set_wss_every)."""
         from .output import write_surface_csv
         folder = os.path.join(outdir, "FlowHemodynamics")
         os.makedirs(folder, exist_ok=True)
