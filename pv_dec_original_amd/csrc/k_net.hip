@@ -255,6 +255,11 @@ __global__ __launch_bounds__(256) void k_conv_fwd_brick(pvnet_conv a, int K) {
 // LDS fills of 128 voxels.  Threads 0..127 each load an 8-voxel x 8-channel block of the gathered
 // input (eight 16-byte loads), transpose it in registers and write eight 16-byte LDS rows of 8
 // voxels; threads 128.. do the same for dy.  The MFMA reduction index is the voxel.
+//
+// The input's LDS rows are swizzled: the 16-byte chunk q of row r sits at chunk q ^ ((r >> 3) & 7).
+// Eight consecutive x-role lanes write rows 8 kc + c (kc = 0..7) at one chunk, and any 16-byte row
+// pitch puts rows eight apart on the same two bank groups (a 4-way conflict); the swizzle spreads
+// them over eight, and the MFMA fragment reads (16 rows, one chunk) stay conflict-free.
 
 // 8x8 transpose of bf16: r[i] = row i (8 channels, channel c in 16-bit half c); o[c] = column c
 // (rows 0..7 in its halves)
@@ -336,7 +341,7 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(pvnet_wgrad a, int M, int K,
         if (xrole || drole) {
             uint4 o[8];
             transpose8(r, o);
-            uint16_t* dst = xrole ? Bs + (kc * 8) * PW + mg * 8 : As + (nc * 8) * PW + mg * 8;
+            uint16_t* dst = xrole ? Bs + (kc * 8) * PW + (mg ^ kc) * 8 : As + (nc * 8) * PW + mg * 8;
 #pragma unroll
             for (int c = 0; c < 8; c++) *(uint4*)(dst + c * PW) = o[c];
         }
@@ -344,7 +349,8 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(pvnet_wgrad a, int M, int K,
         if (mbase + BMW < me) load(mbase + BMW);
 #pragma unroll
         for (int kk = 0; kk < BMW / 32; kk++) {
-            const bf16x8 bx = *(const bf16x8*)(Bs + (wv * 16 + c16) * PW + kk * 32 + g * 8);
+            const int brow = wv * 16 + c16;
+            const bf16x8 bx = *(const bf16x8*)(Bs + brow * PW + ((kk * 4 + g) ^ ((brow >> 3) & 7)) * 8);
 #pragma unroll
             for (int j = 0; j < BN / 16; j++) {
                 const bf16x8 ad = *(const bf16x8*)(As + (j * 16 + c16) * PW + kk * 32 + g * 8);
