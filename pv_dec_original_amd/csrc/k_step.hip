@@ -177,10 +177,14 @@ __device__ __forceinline__ void collide_store_wss(const StepParams& p, int64_t t
         S* dst = at<S>(p.B, D * p.poff_plane + tb, lo);
         if constexpr (NT) __builtin_nontemporal_store(ST::store(fo, Lat<Q>::w[D]), dst);
         else *dst = ST::store(fo, Lat<Q>::w[D]);
-        const R fne = f[D] - feq_dir<Q, D>(mrho, mu);
-        s0 += fne * wss_weight<Q, R, 0, D>(n);
-        s1 += fne * wss_weight<Q, R, 1, D>(n);
-        s2 += fne * wss_weight<Q, R, 2, D>(n);
+        // the rest direction has weight zero: its product is a signed zero, which leaves the
+        // sums unchanged (see wss_weight)
+        if constexpr (Lat<Q>::cx[D] != 0 || Lat<Q>::cy[D] != 0 || Lat<Q>::cz[D] != 0) {
+            const R fne = f[D] - feq_dir<Q, D>(mrho, mu);
+            s0 += fne * wss_weight<Q, R, 0, D>(n);
+            s1 += fne * wss_weight<Q, R, 1, D>(n);
+            s2 += fne * wss_weight<Q, R, 2, D>(n);
+        }
         collide_store_wss<Q, ST, NT, D + 1>(p, tb, lo, rho, u, f, omega, mrho, mu, n, s0, s1, s2);
     }
 }
@@ -260,13 +264,14 @@ __device__ __forceinline__ bool node_update(const StepParams& p, double (&red)[4
     R rho, u[3];
     moments(f, rho, u);
 
-    // WSS launches run every lane through the fused pass (the wave executes it if any lane is
-    // next to the wall); lanes that are not next-to-wall take M = rest, n = 0 and discard the sums.
+    // In WSS launches a tile with a next-to-wall lane runs every lane through the fused pass;
+    // lanes that are not next-to-wall take M = rest, n = 0 and discard the sums.  Tiles without
+    // one (a wave-uniform test) take the plain store.
     const bool ntw = (T.ntw & bit) != 0;
     const int32_t ws = T.wslot + lanes_below(T.ntw);
     R* M = static_cast<R*>(p.M);
     const int64_t ms = p.mst;
-    if (WSS && !init) {
+    if (WSS && !init && T.ntw != 0) {
         R mr = R(1), mu[3] = {R(0), R(0), R(0)}, nv[3] = {R(0), R(0), R(0)};
         if (ntw) {
             mr = M[ws];

@@ -221,18 +221,21 @@ __device__ __forceinline__ void wall_rule(R (&m)[Q], uint32_t nmiss) {
 // Wall shear stress of one node, Simulation::ComputeWallShearStress (Simulation.cpp:882-892):
 // the weight sum_j(A, I) = sum_k c_A,k n_k (c_A,I - sum_l c_A,l n_l n_I) that multiplies the
 // non-equilibrium population of direction A in component I, with the loop's operation order.
+// Terms with a zero velocity component are left out: for a finite normal each is a signed zero,
+// and adding a signed zero to these sums, which start at +0 and so never hold -0, leaves them
+// unchanged, so the weight rounds exactly as the full loop's.
 template <int Q, class R, int I, int A>
 __device__ __forceinline__ R wss_weight(const R (&n)[3]) {
     constexpr int c0 = Lat<Q>::cx[A], c1 = Lat<Q>::cy[A], c2 = Lat<Q>::cz[A];
     constexpr int ci = (I == 0) ? c0 : (I == 1) ? c1 : c2;
     R sum_k = R(0.0);
-    sum_k += R(c0) * n[0] * n[I];
-    sum_k += R(c1) * n[1] * n[I];
-    sum_k += R(c2) * n[2] * n[I];
+    if constexpr (c0 != 0) sum_k += R(c0) * n[0] * n[I];
+    if constexpr (c1 != 0) sum_k += R(c1) * n[1] * n[I];
+    if constexpr (c2 != 0) sum_k += R(c2) * n[2] * n[I];
     R sum_j = R(0.0);
-    sum_j += R(c0) * n[0] * (R(ci) - sum_k);
-    sum_j += R(c1) * n[1] * (R(ci) - sum_k);
-    sum_j += R(c2) * n[2] * (R(ci) - sum_k);
+    if constexpr (c0 != 0) sum_j += R(c0) * n[0] * (R(ci) - sum_k);
+    if constexpr (c1 != 0) sum_j += R(c1) * n[1] * (R(ci) - sum_k);
+    if constexpr (c2 != 0) sum_j += R(c2) * n[2] * (R(ci) - sum_k);
     return sum_j;
 }
 
