@@ -58,7 +58,10 @@ int pvnet_conv_fwd(const pvnet_conv* c, void* stream);
 /* dw[n][k] = sum_m dy[m][n] * x[m + off(tap(k))][ci(k)]  for k < ntaps*cin (row length ntaps*cin)
  * db[n]    = sum_m dy[m][n]                               (if db is not NULL)
  * ws: FP32 scratch of at least pvnet_wgrad_ws_floats(...) floats for the split-K partials; the
- * sums are reduced in a fixed order, so results are deterministic. */
+ * sums are reduced in a fixed order, so results are deterministic.  16-output 3x3x3 layers with at
+ * most 48 input channels on grids of whole 8 x 8 x 4 bricks reduce brick by brick with the input
+ * staged in LDS; flags = PVNET_GATHER keeps them on the per-tap gathering kernel (the same sums in
+ * another order, for comparison).  flags = 0 otherwise. */
 typedef struct {
     pvnet_shape s;
     int32_t ntaps, cin, cout;
@@ -69,6 +72,7 @@ typedef struct {
     float* ws;
     float* dw;
     float* db;
+    int32_t flags;
 } pvnet_wgrad;
 
 int64_t pvnet_wgrad_ws_floats(const pvnet_shape* s, int32_t ntaps, int32_t cin, int32_t cout);

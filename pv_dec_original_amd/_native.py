@@ -65,7 +65,7 @@ class NetWgrad(C.Structure):
     """pvnet_wgrad (weight and bias gradient)."""
     _fields_ = [("s", NetShape), ("ntaps", C.c_int32), ("cin", C.c_int32), ("cout", C.c_int32),
                 ("x", C.c_void_p), ("ldx", C.c_int32), ("lddy", C.c_int32), ("dy", C.c_void_p),
-                ("ws", C.c_void_p), ("dw", C.c_void_p), ("db", C.c_void_p)]
+                ("ws", C.c_void_p), ("dw", C.c_void_p), ("db", C.c_void_p), ("flags", C.c_int32)]
 
 
 NET_RELU, NET_F32OUT, NET_GATHER = 1, 2, 4

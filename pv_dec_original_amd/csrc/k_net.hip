@@ -369,6 +369,152 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(pvnet_wgrad a, int M, int K,
         }
 }
 
+// ---------------------------------------------------------------------------------------------
+// weight gradient of a 16-output 3x3x3 convolution on grids of whole 8 x 8 x 4 bricks.  Block
+// `split` reduces a contiguous run of bricks; per brick it stages dy^T once (16 rows of 256 voxels)
+// and holds its eight MFMA A fragments (32 voxels each) in registers for every tap, then stages
+// the input one 8-channel group at a time as three channel-major copies, one per x shift dx, of
+// the brick's rows with their z/y halo: X[dx][ci][zh][yh] = the 8 voxels x0 + dx + (0..7) of row
+// (z0 - 1 + zh, y0 - 1 + yh), one 16-byte LDS row.  A B fragment (32 voxels x 16 k-columns) is
+// then one aligned 16-byte read per lane.  A 16-column C tile pairs the taps 2p and 2p + 1 of the
+// group's 8 channels (14 tiles, the last half empty); wave w owns tiles w, w + 4, ...
+//
+// LDS rows are 16 bytes; the 64 banks span 16 rows.  The channel stride of 65 rows (= 1 mod 16)
+// puts the 8 channels a half of a 16-lane group reads on 8 consecutive bank groups, and the copy
+// stride of 520 rows (= 8 mod 16) puts the other half, the next tap's copy, on the other 8; dy^T
+// rows are 33 apart (= 1 mod 16) for the same reason.
+
+constexpr int WBX_CH = 65;                 // rows per channel (6 x 10 + 5)
+constexpr int WBX_CPY = 8 * WBX_CH;        // rows per dx copy
+constexpr int WBX_ROWS = 3 * WBX_CPY;      // input rows
+constexpr int WBD_P = 33;                  // dy^T row pitch in 16-byte rows (256 voxels + 1)
+constexpr int WB_SPLITS = 1024;            // blocks (partial rows) at most along the voxels
+constexpr int WB_NGB = 2;                  // 8-channel groups per block
+
+// column c of the 8x8 bf16 block r (rows = voxels, 16-bit halves = channels): 8 voxels of channel c
+template <int c>
+__device__ __forceinline__ uint4 tcol(const uint4 (&r)[8]) {
+    const uint32_t* rw = reinterpret_cast<const uint32_t*>(r);
+    uint32_t o[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const uint32_t lo = rw[(2 * k) * 4 + (c >> 1)], hi = rw[(2 * k + 1) * 4 + (c >> 1)];
+        o[k] = (c & 1) ? ((lo >> 16) | (hi & 0xffff0000u)) : ((lo & 0xffffu) | (hi << 16));
+    }
+    return make_uint4(o[0], o[1], o[2], o[3]);
+}
+
+template <int c = 0>
+__device__ __forceinline__ void store_cols(const uint4 (&r)[8], uint4* dst, int stride) {
+    if constexpr (c < 8) {
+        dst[c * stride] = tcol<c>(r);
+        store_cols<c + 1>(r, dst, stride);
+    }
+}
+
+// block (split, gy): bricks split * per .. ; channel groups gy * WB_NGB + (0 .. NGB - 1)
+template <int NGB>
+__global__ __launch_bounds__(256) void k_conv_wgrad_brick(pvnet_wgrad a, int K, int nbricks, int per) {
+    __shared__ uint4 X[WBX_ROWS];
+    __shared__ uint4 DY[16 * WBD_P];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int g = lane >> 4, c16 = lane & 15;
+    const int d = a.s.d, h = a.s.h, w = a.s.w;
+    const int nbx = w / BRX, nby = h / BRY, nbz = d / BRZ;
+    const int split = blockIdx.x, cg0 = blockIdx.y * WB_NGB;
+    const int ngrp = min(NGB, a.cin / 8 - cg0);
+    const int b0 = split * per, b1 = min(nbricks, b0 + per);
+
+    f32x4 acc[NGB][4];
+#pragma unroll
+    for (int j = 0; j < NGB; j++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) acc[j][i] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    const int cl = c16 & 7;  // this lane's channel within the group; its tap is 2 p + (c16 >> 3)
+
+    for (int b = b0; b < b1; b++) {
+        int bid = b;
+        const int bx = bid % nbx;
+        bid /= nbx;
+        const int by = bid % nby;
+        bid /= nby;
+        const int bz = bid % nbz;
+        const int64_t box = bid / nbz;
+        const int x0 = bx * BRX, y0 = by * BRY, z0 = bz * BRZ;
+
+#pragma unroll
+        for (int j = 0; j < NGB; j++) {
+            if (j >= ngrp) break;  // block-uniform
+            const int cg = cg0 + j;
+            if (j > 0) __syncthreads();  // the previous group's reads of X are done
+            if (t < 180) {
+                // input task (dx, zh, yh): 8 voxels x 8 channels, written as 8 channel rows
+                const int dxi = t / 60, zh = (t / 10) % 6, yh = t % 10;
+                const int z = z0 - 1 + zh, y = y0 - 1 + yh;
+                const bool zy = (unsigned)z < (unsigned)d && (unsigned)y < (unsigned)h;
+                const int64_t row = ((box * d + z) * h + y) * (int64_t)w;
+                uint4 r[8];
+#pragma unroll
+                for (int i = 0; i < 8; i++) {
+                    const int x = x0 + i + dxi - 1;
+                    r[i] = make_uint4(0, 0, 0, 0);
+                    if (zy && (unsigned)x < (unsigned)w) r[i] = *(const uint4*)(a.x + (row + x) * a.ldx + cg * 8);
+                }
+                store_cols(r, X + dxi * WBX_CPY + zh * 10 + yh, WBX_CH);
+            } else if (j == 0 && t >= 192) {
+                // dy task (row r, channel chunk nc): dy^T rows n = 8 nc .. 8 nc + 7 at voxels 8 r ..
+                const int u = t - 192, rr = u >> 1, nc = u & 1;
+                const int64_t row = ((box * d + z0 + (rr >> 3)) * h + y0 + (rr & 7)) * (int64_t)w + x0;
+                uint4 q[8];
+#pragma unroll
+                for (int i = 0; i < 8; i++) q[i] = *(const uint4*)(a.dy + (row + i) * a.lddy + nc * 8);
+                store_cols(q, DY + nc * 8 * WBD_P + rr, WBD_P);
+            }
+            __syncthreads();
+            // A fragments: dy^T[n = c16][voxels 32 s + 8 g .. + 7] of brick row 4 s + g
+            bf16x8 af[8];
+#pragma unroll
+            for (int s2 = 0; s2 < 8; s2++) af[s2] = *reinterpret_cast<const bf16x8*>(&DY[c16 * WBD_P + 4 * s2 + g]);
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int p = wv + 4 * i;
+                if (p >= 14) break;
+                const int tap = 2 * p + (c16 >> 3);
+                const bool valid = tap < 27;
+                int dz, dy, dx;
+                tap_offset(27, valid ? tap : 0, dz, dy, dx);
+                const int base = (dx + 1) * WBX_CPY + cl * WBX_CH + (1 + dz) * 10 + (1 + dy);
+#pragma unroll
+                for (int s2 = 0; s2 < 8; s2++) {
+                    const int r = 4 * s2 + g;  // brick row: z = r / 8, y = r % 8
+                    uint4 v = make_uint4(0, 0, 0, 0);
+                    if (valid) v = X[base + (r >> 3) * 10 + (r & 7)];
+                    acc[j][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[s2], *reinterpret_cast<const bf16x8*>(&v),
+                                                                        acc[j][i], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();  // X and DY are restaged for the next brick
+    }
+
+    // D[4 g + e][c16]: n = 4 g + e, column c16 of tile p -> k = tap * cin + 8 cg + (c16 & 7)
+#pragma unroll
+    for (int j = 0; j < NGB; j++) {
+        if (j >= ngrp) break;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int p = wv + 4 * i;
+            if (p >= 14) break;
+            const int tap = 2 * p + (c16 >> 3);
+            if (tap >= 27) continue;
+            const int k = tap * a.cin + (cg0 + j) * 8 + cl;
+#pragma unroll
+            for (int e = 0; e < 4; e++) a.ws[((int64_t)split * 16 + 4 * g + e) * K + k] = acc[j][i][e];
+        }
+    }
+}
+
 // column sums of dy over a chunk of voxels per block: part[blk][n].  Thread t sums the 8-channel
 // group t % g over the rows r = t / g, t / g + R, ... (g = cout / 8, R = 256 / g); the R partial
 // rows are then added in ascending order.
@@ -395,13 +541,25 @@ __global__ __launch_bounds__(256) void k_colsum(const uint16_t* dy, int ld, int 
     }
 }
 
-// dst[i] = sum_s src[s * len + i], s ascending
-__global__ void k_sum_splits(const float* src, int nsplit, int64_t len, float* dst, int64_t ldd, int64_t rowlen) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= len) return;
+// dst[i] = sum_s src[s * len + i] in a fixed order: a block takes 16 columns, its 16 row groups
+// sum the rows rg, rg + 16, ... in ascending order, and the 16 partials are added in ascending rg
+constexpr int SUM_COLS = 16;
+
+__global__ __launch_bounds__(256) void k_sum_splits(const float* src, int nsplit, int64_t len, float* dst,
+                                                    int64_t ldd, int64_t rowlen) {
+    __shared__ float sp[16][SUM_COLS + 1];
+    const int t = threadIdx.x, col = t % SUM_COLS, rg = t / SUM_COLS;
+    const int64_t i = (int64_t)blockIdx.x * SUM_COLS + col;
     float s = 0.f;
-    for (int p = 0; p < nsplit; p++) s += src[(int64_t)p * len + i];
-    dst[(i / rowlen) * ldd + i % rowlen] = s;
+    if (i < len)
+        for (int p = rg; p < nsplit; p += 16) s += src[(int64_t)p * len + i];
+    sp[rg][col] = s;
+    __syncthreads();
+    if (t < SUM_COLS && i < len) {
+        float a = 0.f;
+        for (int r = 0; r < 16; r++) a += sp[r][t];
+        dst[(i / rowlen) * ldd + i % rowlen] = a;
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -701,11 +859,20 @@ int kpad_of(int ntaps, int c) { return (ntaps * c + BK - 1) / BK * BK; }
 struct WgradPlan {
     int ktiles, ntiles, bn, nsplit, mchunk, cblk, cchunk;
     int64_t ws_main, ws_total;
+    bool brick;        // k_conv_wgrad_brick (nsplit blocks of `per` bricks each)
+    int nbricks, per;
 };
+
+// the brick kernel takes 16-output 3x3x3 layers with up to 48 input channels on grids of whole
+// bricks; PVNET_GATHER keeps them on k_conv_wgrad
+bool wgrad_brick_ok(const pvnet_shape& s, int ntaps, int cin, int cout) {
+    return ntaps == 27 && cout == 16 && cin % 8 == 0 && cin / 8 >= 1 && cin / 8 <= 6 && s.d % BRZ == 0 &&
+           s.h % BRY == 0 && s.w % BRX == 0;
+}
 
 int wgrad_bn(int cout) { return cout <= 16 ? 16 : (cout <= 32 || cout % 64 != 0) ? 32 : 64; }
 
-WgradPlan plan_wgrad(const pvnet_shape& s, int ntaps, int cin, int cout) {
+WgradPlan plan_wgrad(const pvnet_shape& s, int ntaps, int cin, int cout, int flags) {
     WgradPlan p{};
     const int64_t M = voxels(s);
     const int K = ntaps * cin;
@@ -719,6 +886,14 @@ WgradPlan plan_wgrad(const pvnet_shape& s, int ntaps, int cin, int cout) {
     p.mchunk = (int)(per * 128);
     p.nsplit = (int)((M + p.mchunk - 1) / p.mchunk);
     p.ws_main = (int64_t)p.nsplit * cout * K;
+    p.brick = !(flags & PVNET_GATHER) && wgrad_brick_ok(s, ntaps, cin, cout);
+    if (p.brick) {
+        p.nbricks = (int)(M / (BRX * BRY * BRZ));
+        const int ns2 = std::min(p.nbricks, WB_SPLITS);
+        p.per = (p.nbricks + ns2 - 1) / ns2;
+        p.nsplit = (p.nbricks + p.per - 1) / p.per;
+        p.ws_main = (int64_t)p.nsplit * cout * K;
+    }
     p.cchunk = 8192;
     p.cblk = (int)((M + p.cchunk - 1) / p.cchunk);
     p.ws_total = p.ws_main + (int64_t)p.cblk * cout;
@@ -774,7 +949,8 @@ int pvnet_conv_fwd(const pvnet_conv* c, void* stream) {
 
 int64_t pvnet_wgrad_ws_floats(const pvnet_shape* s, int32_t ntaps, int32_t cin, int32_t cout) {
     if (!shape_ok(s) || cin <= 0 || cout <= 0) return -1;
-    return plan_wgrad(*s, ntaps, cin, cout).ws_total;
+    // either kernel's partials fit (the call's flags pick one)
+    return std::max(plan_wgrad(*s, ntaps, cin, cout, 0).ws_total, plan_wgrad(*s, ntaps, cin, cout, PVNET_GATHER).ws_total);
 }
 
 int pvnet_conv_wgrad(const pvnet_wgrad* a, void* stream) {
@@ -789,10 +965,27 @@ int pvnet_conv_wgrad(const pvnet_wgrad* a, void* stream) {
     if (M % 32) return fail(1, "pvnet_conv_wgrad: voxel count must be a multiple of 32");
     if (M >= (int64_t)1 << 31) return fail(1, "pvnet_conv_wgrad: too many voxels");
     hipStream_t st = (hipStream_t)stream;
-    const WgradPlan p = plan_wgrad(a->s, a->ntaps, a->cin, a->cout);
+    const WgradPlan p = plan_wgrad(a->s, a->ntaps, a->cin, a->cout, a->flags);
     const int K = a->ntaps * a->cin;
     dim3 grid(p.ktiles, p.ntiles, p.nsplit);
-    if (p.bn == 16)
+    int nsplit = p.nsplit;
+    if (p.brick) {
+        // one round of resident blocks along the voxels (at most the plan's WB_SPLITS partial rows)
+        const int ng = a->cin / 8;
+        const void* kern = ng == 1 ? (const void*)k_conv_wgrad_brick<1> : (const void*)k_conv_wgrad_brick<WB_NGB>;
+        int dev = 0, cus = 0, occ = 0;
+        NET_HIP(hipGetDevice(&dev));
+        NET_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        NET_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, 256, 0));
+        const int want = std::max(1, std::min({p.nbricks, WB_SPLITS, std::max(1, cus * occ)}));
+        const int per = (p.nbricks + want - 1) / want;
+        nsplit = (p.nbricks + per - 1) / per;
+        const dim3 gb(nsplit, (ng + WB_NGB - 1) / WB_NGB);
+        if (ng == 1)
+            hipLaunchKernelGGL(k_conv_wgrad_brick<1>, gb, dim3(256), 0, st, *a, K, p.nbricks, per);
+        else
+            hipLaunchKernelGGL(k_conv_wgrad_brick<WB_NGB>, gb, dim3(256), 0, st, *a, K, p.nbricks, per);
+    } else if (p.bn == 16)
         hipLaunchKernelGGL(k_conv_wgrad<16>, grid, dim3(256), 0, st, *a, (int)M, K, p.mchunk);
     else if (p.bn == 32)
         hipLaunchKernelGGL(k_conv_wgrad<32>, grid, dim3(256), 0, st, *a, (int)M, K, p.mchunk);
@@ -800,7 +993,7 @@ int pvnet_conv_wgrad(const pvnet_wgrad* a, void* stream) {
         hipLaunchKernelGGL(k_conv_wgrad<64>, grid, dim3(256), 0, st, *a, (int)M, K, p.mchunk);
     NET_LAUNCH();
     const int64_t len = (int64_t)a->cout * K;
-    hipLaunchKernelGGL(k_sum_splits, dim3(blocks_for(len, 256)), dim3(256), 0, st, a->ws, p.nsplit, len, a->dw,
+    hipLaunchKernelGGL(k_sum_splits, dim3(blocks_for(len, SUM_COLS)), dim3(256), 0, st, a->ws, nsplit, len, a->dw,
                        (int64_t)K, (int64_t)K);
     NET_LAUNCH();
     if (a->db) {
@@ -808,8 +1001,8 @@ int pvnet_conv_wgrad(const pvnet_wgrad* a, void* stream) {
         float* part = a->ws + p.ws_main;
         hipLaunchKernelGGL(k_colsum, dim3(p.cblk), dim3(256), 0, st, a->dy, a->lddy, a->cout, (int)M, p.cchunk, part);
         NET_LAUNCH();
-        hipLaunchKernelGGL(k_sum_splits, dim3(1), dim3(256), 0, st, part, p.cblk, (int64_t)a->cout, a->db,
-                           (int64_t)a->cout, (int64_t)a->cout);  // cblk <= a few hundred partial rows
+        hipLaunchKernelGGL(k_sum_splits, dim3(blocks_for(a->cout, SUM_COLS)), dim3(256), 0, st, part, p.cblk,
+                           (int64_t)a->cout, a->db, (int64_t)a->cout, (int64_t)a->cout);
         NET_LAUNCH();
     }
     return 0;

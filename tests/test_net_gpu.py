@@ -202,6 +202,42 @@ def test_conv_fwd_brick_equals_gather(T, cin, flags, ldx, xoff, ldy, yoff, mask)
     assert a[:, yoff:yoff + cout].abs().max().item() > 0
 
 
+@pytest.mark.parametrize("shape,cin,ldx,xoff", [
+    ((2, 8, 16, 16), 8, 8, 0), ((1, 12, 16, 24), 16, 24, 8), ((3, 4, 8, 8), 24, 24, 0),
+    ((2, 8, 8, 16), 48, 56, 8)])
+def test_wgrad_brick_against_oracle_and_gather(T, shape, cin, ldx, xoff):
+    """16-output 3x3x3 weight gradients on grids of whole 8 x 8 x 4 bricks reduce brick by brick
+    with the input staged in LDS: against the FP64 oracle (FP32 accumulation, 1e-5 of the scale)
+    and against the per-tap gathering kernel (PVNET_GATHER, the same sums in another order), over
+    several boxes (zero padding at every face), one to six 8-channel groups, and an input slice of
+    a wider tensor; two runs give the same bits."""
+    from pv_dec_original_amd import _native as N
+    NL = N.net()
+    cout = 16
+    M = O.nvox(shape)
+    xfull = np.zeros((M, ldx))
+    xfull[:, xoff:xoff + cin] = _rbf((M, cin))
+    dy = _rbf((M, cout))
+    refdw, refdb = O.conv_wgrad(shape, 27, cin, cout, xfull[:, xoff:xoff + cin], dy)
+    xd, dyd = _dev(T, xfull), _dev(T, dy)
+    ws = T.zeros(int(NL.pvnet_wgrad_ws_floats(C.byref(N.NetShape(*shape)), 27, cin, cout)), dtype=T.float32,
+                 device="cuda")
+    sc = np.abs(refdw).max()
+    out = {}
+    for run, flags in (("brick", 0), ("brick again", 0), ("gather", N.NET_GATHER)):
+        dw = T.full((cout, 27 * cin), 7.0, dtype=T.float32, device="cuda")
+        db = T.zeros(cout, dtype=T.float32, device="cuda")
+        g = N.NetWgrad(N.NetShape(*shape), 27, cin, cout, _p(xd, xoff), ldx, cout, _p(dyd), _p(ws), _p(dw), _p(db),
+                       flags)
+        N.net_check(NL.pvnet_conv_wgrad(C.byref(g), _st(T)))
+        T.cuda.synchronize()
+        out[run] = _np(dw)
+        assert np.allclose(out[run], refdw, rtol=1e-5, atol=1e-5 * sc), (run, np.abs(out[run] - refdw).max() / sc)
+        assert np.allclose(_np(db), refdb, rtol=1e-5, atol=1e-5 * np.abs(refdb).max())
+    assert np.array_equal(out["brick"], out["brick again"])
+    assert np.allclose(out["brick"], out["gather"], rtol=1e-5, atol=1e-6 * sc)
+
+
 def test_wgrad_deterministic(T):
     from pv_dec_original_amd import _native as N
     NL = N.net()
