@@ -36,9 +36,9 @@ enum { PVNET_RELU = 1, PVNET_F32OUT = 2, PVNET_GATHER = 4 };
  * ntaps*cin.  If relu_of is not NULL the result is multiplied by (relu_of[m][n] > 0) (the
  * backward of a ReLU whose output is relu_of).  flags: PVNET_RELU applies max(., 0);
  * PVNET_F32OUT writes float instead of bf16.  y may be a channel slice of a wider tensor.
- * 16-output 3x3x3 convolutions on grids of whole 8 x 8 x 4 bricks stage each brick and its halo
- * in LDS once; PVNET_GATHER keeps them on the per-tap gathering kernel (the same result bit for
- * bit, for comparison). */
+ * 16- and 32-output 3x3x3 convolutions on grids of whole 16 x 4 x 4 bricks (x, y, z) stage each
+ * brick and its halo in LDS once; PVNET_GATHER keeps them on the per-tap gathering kernel (the
+ * same result bit for bit, for comparison). */
 typedef struct {
     pvnet_shape s;
     int32_t ntaps, cin, cout, kpad;

@@ -29,6 +29,12 @@ LIBS = {
     "libpvnet.so": ["k_net.hip"],
 }
 
+# per-library flags: the network's MFMA accumulators stay in VGPRs (the AGPR form made the compiler
+# copy every accumulator between the two register files around each MFMA of the k-loops)
+LIB_FLAGS = {
+    "libpvnet.so": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
+}
+
 
 def _run(cmd, verbose):
     if verbose:
@@ -45,7 +51,8 @@ def build(verbose=False):
     for lib, srcs in LIBS.items():
         out = os.path.join(HERE, lib)
         tmp = out + ".tmp"
-        _run([hipcc] + HIP_FLAGS + [os.path.join(CSRC, s) for s in srcs] + ["-o", tmp], verbose)
+        _run([hipcc] + HIP_FLAGS + LIB_FLAGS.get(lib, []) + [os.path.join(CSRC, s) for s in srcs] + ["-o", tmp],
+             verbose)
         os.replace(tmp, out)
     gen = os.path.join(HERE, "libpvgen.so")
     _run(["g++", "-O2", "-fopenmp", "-fPIC", "-shared", "-std=c++17", "-Wall",

@@ -62,6 +62,16 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 // bf16 bit pattern strictly greater than zero (NaN is not)
 __device__ __forceinline__ bool bf_pos(uint16_t b) { return (b & 0x8000u) == 0 && b != 0 && b <= 0x7f80u; }
 
+// Workgroups are dispatched to the 8 XCDs round-robin (block b to XCD b % 8), each XCD with its own
+// L2.  Neighbouring output tiles share most of their input (the 3x3x3 halo), so block b works on
+// tile xcd_tile(b, n): XCD x takes a contiguous run of tiles, and neighbours meet in one L2.
+constexpr int NXCD = 8;
+
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+    const int q = n / NXCD, r = n % NXCD, x = b % NXCD, i = b / NXCD;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
+}
+
 __device__ __forceinline__ void tap_offset(int ntaps, int tap, int& dz, int& dy, int& dx) {
     if (ntaps == 27) {
         dz = tap / 9 - 1;
@@ -83,7 +93,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd(pvnet_conv a, int M, int K) {
     __shared__ __attribute__((aligned(16))) uint16_t Bs[BN * LP];
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int g = lane >> 4, c16 = lane & 15;
-    const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+    const int m0 = xcd_tile(blockIdx.x, gridDim.x) * BM, n0 = blockIdx.y * BN;
     const int d = a.s.d, h = a.s.h, w = a.s.w;
 
     // load roles: A rows r + 64 i, 8-wide k chunk q; B row bn (if < BN), same chunk
@@ -171,23 +181,30 @@ __global__ __launch_bounds__(256) void k_conv_fwd(pvnet_conv a, int M, int K) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// 3x3x3 convolution with 16 output channels on grids of whole 8 x 8 x 4 bricks: each block stages
-// one brick of one box with its one-voxel halo (10 x 10 x 6 voxels, all cin channels) in LDS once
-// and runs the k-steps of k_conv_fwd over it, where k_conv_fwd gathers every tap's rows from L2
-// again.  The k-steps, their operands and their order are those of k_conv_fwd, so each output is
-// the same bit for bit.  Wave wv computes brick plane z = wv as 4 MFMA row tiles of 16 voxels
-// (x = c % 8, y = 2 tile + c / 8).
+// 3x3x3 convolution with 16 or 32 output channels (NT column tiles) on grids of whole 16 x 4 x 4
+// bricks: each block stages one brick of one box with its one-voxel halo (18 x 6 x 6 voxels, all
+// cin channels) in LDS once and runs the k-steps of k_conv_fwd over it, where k_conv_fwd gathers
+// every tap's rows from L2 again.  The k-steps, their operands and their order are those of
+// k_conv_fwd, so each output is the same bit for bit.  Wave wv computes brick plane z = wv as 4
+// MFMA row tiles (y = tile) of 16 voxels (x = MFMA row).  The halo is stored chunk-major
+// (halo[chunk][voxel], a chunk = 8 channels, 16 bytes), so the 16 lanes that read one A fragment
+// row group address 16 consecutive 16-byte rows: one pass over the 64 banks.
 
-constexpr int BRX = 8, BRY = 8, BRZ = 4;
+constexpr int BRX = 16, BRY = 4, BRZ = 4;
 constexpr int HLX = BRX + 2, HLY = BRY + 2, HLZ = BRZ + 2, HLV = HLX * HLY * HLZ;
 
+// step table: entry 4 s + g = the halo offset of k-values s * 32 + 8 g .. + 7 relative to the
+// output voxel (chunk * HLV + tap shift), plus BR_TAB_BIAS; 0 where they lie beyond K
+constexpr int BR_TAB_BIAS = 1 << 20;
+
+template <int NT>
 __global__ __launch_bounds__(256) void k_conv_fwd_brick(pvnet_conv a, int K) {
-    extern __shared__ uint4 halo[];  // [HLV voxels][cin / 8 chunks of 8 bf16]
+    extern __shared__ uint4 halo[];  // [cin / 8 chunks][HLV voxels], then the step table
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int g = lane >> 4, c16 = lane & 15;
     const int d = a.s.d, h = a.s.h, w = a.s.w;
     const int nbx = w / BRX, nby = h / BRY, nbz = d / BRZ;
-    int bid = blockIdx.x;
+    int bid = xcd_tile(blockIdx.x, gridDim.x);
     const int bx = bid % nbx;
     bid /= nbx;
     const int by = bid % nby;
@@ -196,8 +213,21 @@ __global__ __launch_bounds__(256) void k_conv_fwd_brick(pvnet_conv a, int K) {
     const int box = bid / nbz;
     const int x0 = bx * BRX, y0 = by * BRY, z0 = bz * BRZ;
     const int cch = a.cin >> 3;
+    const int nsteps = a.kpad / BK;
+    int* tab = reinterpret_cast<int*>(halo + HLV * cch);
+    for (int i = t; i < 4 * nsteps; i += 256) {
+        const int k = (i >> 2) * BK + (i & 3) * 8;
+        int v = 0;
+        if (k < K) {
+            const int tap = k / a.cin, ci = k - tap * a.cin;
+            int dz, dy, dx;
+            tap_offset(27, tap, dz, dy, dx);
+            v = (ci >> 3) * HLV + (dz * HLY + dy) * HLX + dx + BR_TAB_BIAS;
+        }
+        tab[i] = v;
+    }
     for (int i = t; i < HLV * cch; i += 256) {
-        const int v = i / cch, ch = i - v * cch;
+        const int ch = i / HLV, v = i - ch * HLV;
         const int x = x0 - 1 + v % HLX, y = y0 - 1 + (v / HLX) % HLY, z = z0 - 1 + v / (HLX * HLY);
         uint4 val = make_uint4(0, 0, 0, 0);
         if ((unsigned)x < (unsigned)w && (unsigned)y < (unsigned)h && (unsigned)z < (unsigned)d) {
@@ -208,46 +238,74 @@ __global__ __launch_bounds__(256) void k_conv_fwd_brick(pvnet_conv a, int K) {
     }
     __syncthreads();
 
-    f32x4 acc[4];
+    f32x4 acc[4][NT];
 #pragma unroll
-    for (int i = 0; i < 4; i++) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
-    const int xl = c16 & 7, yl = c16 >> 3;
-    const int nsteps = a.kpad / BK;
-    for (int s = 0; s < nsteps; s++) {
-        const int k = s * BK + g * 8;
-        const bool kin = k < K;
-        const int tap = kin ? k / a.cin : 0;
-        const int ci = k - tap * a.cin;
-        int dz, dy, dx;
-        tap_offset(27, tap, dz, dy, dx);
-        const bf16x8 bfr = *(const bf16x8*)(a.w + (int64_t)c16 * a.kpad + s * BK + g * 8);
-        // halo voxel of (xl, 2 i + yl, wv) shifted by the tap
-        const int base = ((wv + 1 + dz) * HLY + (yl + 1 + dy)) * HLX + (xl + 1 + dx);
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < NT; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // halo voxel of (c16, tile 0, wv); tile i adds i rows.  Only the last step can reach beyond K
+    // (kpad - K < 32), so the others run without a check, each prefetching the next step's table
+    // entry and weight fragments.
+    const int vox = ((wv + 1) * HLY + 1) * HLX + c16 + 1 - BR_TAB_BIAS;
+    const uint16_t* wrow[NT];
+#pragma unroll
+    for (int j = 0; j < NT; j++) wrow[j] = a.w + (int64_t)(16 * j + c16) * a.kpad + g * 8;
+    int tv = tab[g];
+    bf16x8 bfr[NT];
+#pragma unroll
+    for (int j = 0; j < NT; j++) bfr[j] = *(const bf16x8*)(wrow[j]);
+    for (int s = 0; s + 1 < nsteps; s++) {
+        const int tvn = tab[4 * (s + 1) + g];
+        bf16x8 bn[NT];
+#pragma unroll
+        for (int j = 0; j < NT; j++) bn[j] = *(const bf16x8*)(wrow[j] + (s + 1) * BK);
+        const int base = vox + tv;
+        uint4 r[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) r[i] = halo[base + i * HLX];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < NT; j++)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(*reinterpret_cast<const bf16x8*>(&r[i]), bfr[j],
+                                                                    acc[i][j], 0, 0, 0);
+        tv = tvn;
+#pragma unroll
+        for (int j = 0; j < NT; j++) bfr[j] = bn[j];
+    }
+    {
+        const int base = vox + tv;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             uint4 r = make_uint4(0, 0, 0, 0);
-            if (kin) r = halo[(base + 2 * i * HLX) * cch + (ci >> 3)];
-            acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(*reinterpret_cast<const bf16x8*>(&r), bfr, acc[i], 0, 0, 0);
+            if (tv) r = halo[base + i * HLX];
+#pragma unroll
+            for (int j = 0; j < NT; j++)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(*reinterpret_cast<const bf16x8*>(&r), bfr[j],
+                                                                    acc[i][j], 0, 0, 0);
         }
     }
 
     const bool relu = a.flags & PVNET_RELU, f32out = a.flags & PVNET_F32OUT;
-    const int n = c16;
-    const float bias = a.bias ? a.bias[n] : 0.f;
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int j = 0; j < NT; j++) {
+        const int n = 16 * j + c16;
+        const float bias = a.bias ? a.bias[n] : 0.f;
 #pragma unroll
-        for (int e = 0; e < 4; e++) {
-            const int r = 4 * g + e;  // MFMA row: voxel (r % 8, 2 i + r / 8) of plane wv
-            const int64_t m = (((int64_t)box * d + z0 + wv) * h + y0 + 2 * i + (r >> 3)) * w + x0 + (r & 7);
-            float v = acc[i][e] + bias;
-            if (relu) v = v < 0.f ? 0.f : v;
-            if (a.relu_of && !bf_pos(a.relu_of[m * a.ldr + n])) v = 0.f;
-            if (f32out)
-                ((float*)a.y)[m * a.ldy + n] = v;
-            else
-                ((uint16_t*)a.y)[m * a.ldy + n] = f2bf(v);
-        }
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                // MFMA row 4 g + e: voxel x0 + 4 g + e of row y0 + i in plane z0 + wv
+                const int64_t m = (((int64_t)box * d + z0 + wv) * h + y0 + i) * w + x0 + 4 * g + e;
+                float v = acc[i][j][e] + bias;
+                if (relu) v = v < 0.f ? 0.f : v;
+                if (a.relu_of && !bf_pos(a.relu_of[m * a.ldr + n])) v = 0.f;
+                if (f32out)
+                    ((float*)a.y)[m * a.ldy + n] = v;
+                else
+                    ((uint16_t*)a.y)[m * a.ldy + n] = f2bf(v);
+            }
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -384,9 +442,11 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(pvnet_wgrad a, int M, int K,
 // stride of 520 rows (= 8 mod 16) puts the other half, the next tap's copy, on the other 8; dy^T
 // rows are 33 apart (= 1 mod 16) for the same reason.
 
+constexpr int WBRX = 8, WBRY = 8, WBRZ = 4;  // weight-gradient bricks
 constexpr int WBX_CH = 65;                 // rows per channel (6 x 10 + 5)
 constexpr int WBX_CPY = 8 * WBX_CH;        // rows per dx copy
 constexpr int WBX_ROWS = 3 * WBX_CPY;      // input rows
+constexpr int WBX_ZERO = 40;               // zero rows after them (row offsets within a tile <= 37)
 constexpr int WBD_P = 33;                  // dy^T row pitch in 16-byte rows (256 voxels + 1)
 constexpr int WB_SPLITS = 1024;            // blocks (partial rows) at most along the voxels
 constexpr int WB_NGB = 2;                  // 8-channel groups per block
@@ -415,12 +475,13 @@ __device__ __forceinline__ void store_cols(const uint4 (&r)[8], uint4* dst, int 
 // block (split, gy): bricks split * per .. ; channel groups gy * WB_NGB + (0 .. NGB - 1)
 template <int NGB>
 __global__ __launch_bounds__(256) void k_conv_wgrad_brick(pvnet_wgrad a, int K, int nbricks, int per) {
-    __shared__ uint4 X[WBX_ROWS];
+    // X ends in WBX_ZERO zero rows: the lanes of the empty half of the last tap pair read there
+    __shared__ uint4 X[WBX_ROWS + WBX_ZERO];
     __shared__ uint4 DY[16 * WBD_P];
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int g = lane >> 4, c16 = lane & 15;
     const int d = a.s.d, h = a.s.h, w = a.s.w;
-    const int nbx = w / BRX, nby = h / BRY, nbz = d / BRZ;
+    const int nbx = w / WBRX, nby = h / WBRY, nbz = d / WBRZ;
     const int split = blockIdx.x, cg0 = blockIdx.y * WB_NGB;
     const int ngrp = min(NGB, a.cin / 8 - cg0);
     const int b0 = split * per, b1 = min(nbricks, b0 + per);
@@ -432,6 +493,7 @@ __global__ __launch_bounds__(256) void k_conv_wgrad_brick(pvnet_wgrad a, int K, 
         for (int i = 0; i < 4; i++) acc[j][i] = f32x4{0.f, 0.f, 0.f, 0.f};
 
     const int cl = c16 & 7;  // this lane's channel within the group; its tap is 2 p + (c16 >> 3)
+    if (t < WBX_ZERO) X[WBX_ROWS + t] = make_uint4(0, 0, 0, 0);  // ordered by the first barrier
 
     for (int b = b0; b < b1; b++) {
         int bid = b;
@@ -441,7 +503,7 @@ __global__ __launch_bounds__(256) void k_conv_wgrad_brick(pvnet_wgrad a, int K, 
         bid /= nby;
         const int bz = bid % nbz;
         const int64_t box = bid / nbz;
-        const int x0 = bx * BRX, y0 = by * BRY, z0 = bz * BRZ;
+        const int x0 = bx * WBRX, y0 = by * WBRY, z0 = bz * WBRZ;
 
 #pragma unroll
         for (int j = 0; j < NGB; j++) {
@@ -481,15 +543,13 @@ __global__ __launch_bounds__(256) void k_conv_wgrad_brick(pvnet_wgrad a, int K, 
                 const int p = wv + 4 * i;
                 if (p >= 14) break;
                 const int tap = 2 * p + (c16 >> 3);
-                const bool valid = tap < 27;
                 int dz, dy, dx;
-                tap_offset(27, valid ? tap : 0, dz, dy, dx);
-                const int base = (dx + 1) * WBX_CPY + cl * WBX_CH + (1 + dz) * 10 + (1 + dy);
+                tap_offset(27, tap < 27 ? tap : 0, dz, dy, dx);
+                const int base = tap < 27 ? (dx + 1) * WBX_CPY + cl * WBX_CH + (1 + dz) * 10 + (1 + dy) : WBX_ROWS;
 #pragma unroll
                 for (int s2 = 0; s2 < 8; s2++) {
                     const int r = 4 * s2 + g;  // brick row: z = r / 8, y = r % 8
-                    uint4 v = make_uint4(0, 0, 0, 0);
-                    if (valid) v = X[base + (r >> 3) * 10 + (r & 7)];
+                    const uint4 v = X[base + (r >> 3) * 10 + (r & 7)];
                     acc[j][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[s2], *reinterpret_cast<const bf16x8*>(&v),
                                                                         acc[j][i], 0, 0, 0);
                 }
@@ -866,8 +926,8 @@ struct WgradPlan {
 // the brick kernel takes 16-output 3x3x3 layers with up to 48 input channels on grids of whole
 // bricks; PVNET_GATHER keeps them on k_conv_wgrad
 bool wgrad_brick_ok(const pvnet_shape& s, int ntaps, int cin, int cout) {
-    return ntaps == 27 && cout == 16 && cin % 8 == 0 && cin / 8 >= 1 && cin / 8 <= 6 && s.d % BRZ == 0 &&
-           s.h % BRY == 0 && s.w % BRX == 0;
+    return ntaps == 27 && cout == 16 && cin % 8 == 0 && cin / 8 >= 1 && cin / 8 <= 6 && s.d % WBRZ == 0 &&
+           s.h % WBRY == 0 && s.w % WBRX == 0;
 }
 
 int wgrad_bn(int cout) { return cout <= 16 ? 16 : (cout <= 32 || cout % 64 != 0) ? 32 : 64; }
@@ -888,7 +948,7 @@ WgradPlan plan_wgrad(const pvnet_shape& s, int ntaps, int cin, int cout, int fla
     p.ws_main = (int64_t)p.nsplit * cout * K;
     p.brick = !(flags & PVNET_GATHER) && wgrad_brick_ok(s, ntaps, cin, cout);
     if (p.brick) {
-        p.nbricks = (int)(M / (BRX * BRY * BRZ));
+        p.nbricks = (int)(M / (WBRX * WBRY * WBRZ));
         const int ns2 = std::min(p.nbricks, WB_SPLITS);
         p.per = (p.nbricks + ns2 - 1) / ns2;
         p.nsplit = (p.nbricks + p.per - 1) / p.per;
@@ -930,10 +990,14 @@ int pvnet_conv_fwd(const pvnet_conv* c, void* stream) {
     // tiles: 256-voxel blocks for narrow outputs on large grids; 64-channel tiles only when the
     // grid still has >= 2048 blocks (the K loop is long, so the device needs many blocks in flight)
     const int mt = blocks_for(M, 128);
-    const size_t halo_bytes = (size_t)HLV * c->cin * 2;
-    if (c->cout == 16 && c->ntaps == 27 && !(c->flags & PVNET_GATHER) && c->s.d % BRZ == 0 &&
+    const size_t halo_bytes = (size_t)HLV * c->cin * 2 + (size_t)(c->kpad / BK) * 4 * sizeof(int);
+    if ((c->cout == 16 || c->cout == 32) && c->ntaps == 27 && !(c->flags & PVNET_GATHER) && c->s.d % BRZ == 0 &&
         c->s.h % BRY == 0 && c->s.w % BRX == 0 && halo_bytes <= 64 * 1024 && M >= (int64_t)256 * 2048) {
-        hipLaunchKernelGGL(k_conv_fwd_brick, dim3((unsigned)(M / (BRX * BRY * BRZ))), dim3(256), halo_bytes, st, *c, K);
+        const dim3 gb((unsigned)(M / (BRX * BRY * BRZ)));
+        if (c->cout == 16)
+            hipLaunchKernelGGL(k_conv_fwd_brick<1>, gb, dim3(256), halo_bytes, st, *c, K);
+        else
+            hipLaunchKernelGGL(k_conv_fwd_brick<2>, gb, dim3(256), halo_bytes, st, *c, K);
     } else if (c->cout <= 16 && M >= (int64_t)256 * 2048) {
         hipLaunchKernelGGL((k_conv_fwd<16, 64>), dim3(blocks_for(M, 256), 1), dim3(256), 0, st, *c, (int)M, K);
     } else if (c->cout <= 16) {

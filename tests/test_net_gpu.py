@@ -132,8 +132,9 @@ def test_conv_fwd_and_dgrad_and_wgrad(T, case):
 
 
 def test_conv_fwd_large_grid_tiles(T):
-    """The 256-voxel tile used for narrow outputs on large grids, on sampled voxels against a direct
-    numpy sum over the 27 taps."""
+    """The 256-voxel gathering tile used for narrow outputs on large grids (PVNET_GATHER keeps the
+    convolution off the brick kernel), on sampled voxels against a direct numpy sum over the 27
+    taps."""
     from pv_dec_original_amd import _native as N
     NL = N.net()
     shape, cin, cout = (1, 64, 96, 96), 16, 16
@@ -147,7 +148,8 @@ def test_conv_fwd_large_grid_tiles(T):
     wd = T.zeros((cin, (27 * cout + 31) // 32 * 32), dtype=T.bfloat16, device="cuda")
     N.net_check(NL.pvnet_pack_weights(_p(w32), 27, cin, cout, _p(wt), _p(wd), _st(T)))
     yd = T.zeros((M, cout), dtype=T.bfloat16, device="cuda")
-    c = N.NetConv(N.NetShape(*shape), 27, cin, cout, kpad, _p(xd), cin, 1, _p(wt), _p(bd), _p(yd), cout, 0, None)
+    c = N.NetConv(N.NetShape(*shape), 27, cin, cout, kpad, _p(xd), cin, 1 | N.NET_GATHER, _p(wt), _p(bd), _p(yd), cout,
+                  0, None)
     N.net_check(NL.pvnet_conv_fwd(C.byref(c), _st(T)))
     T.cuda.synchronize()
     got = _np(yd)
@@ -165,17 +167,18 @@ def test_conv_fwd_large_grid_tiles(T):
         _close_bf16(got[m], ref, f"voxel {m}")
 
 
-@pytest.mark.parametrize("cin,flags,ldx,xoff,ldy,yoff,mask", [
-    (8, 1, 8, 0, 16, 0, False), (16, 1, 24, 8, 48, 32, False), (32, 2, 32, 0, 16, 0, False),
-    (48, 0, 48, 0, 16, 0, True)])
-def test_conv_fwd_brick_equals_gather(T, cin, flags, ldx, xoff, ldy, yoff, mask):
-    """16-output 3x3x3 convolutions on whole 8 x 8 x 4 bricks stage each brick with its halo in
-    LDS; the result must equal the per-tap gathering kernel's (PVNET_GATHER) bit for bit, over two
-    boxes (zero padding at every box face), channel slices of wider tensors, ReLU, FP32 output and
-    the ReLU-backward mask.  Columns outside the output slice stay untouched."""
+@pytest.mark.parametrize("cin,cout,flags,ldx,xoff,ldy,yoff,mask", [
+    (8, 16, 1, 8, 0, 16, 0, False), (16, 16, 1, 24, 8, 48, 32, False), (32, 16, 2, 32, 0, 16, 0, False),
+    (48, 16, 0, 48, 0, 16, 0, True), (16, 32, 1, 16, 0, 64, 32, False), (32, 32, 0, 40, 8, 32, 0, True),
+    (48, 32, 3, 48, 0, 32, 0, False)])
+def test_conv_fwd_brick_equals_gather(T, cin, cout, flags, ldx, xoff, ldy, yoff, mask):
+    """16- and 32-output 3x3x3 convolutions on whole 16 x 4 x 4 bricks stage each brick with its
+    halo in LDS; the result must equal the per-tap gathering kernel's (PVNET_GATHER) bit for bit,
+    over two boxes (zero padding at every box face), channel slices of wider tensors, ReLU, FP32
+    output and the ReLU-backward mask.  Columns outside the output slice stay untouched."""
     from pv_dec_original_amd import _native as N
     NL = N.net()
-    shape, cout = (2, 32, 96, 96), 16
+    shape = (2, 32, 96, 96)
     M = O.nvox(shape)
     xfull = np.zeros((M, ldx))
     xfull[:, xoff:xoff + cin] = _rbf((M, cin))
