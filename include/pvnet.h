@@ -36,8 +36,8 @@ enum { PVNET_RELU = 1, PVNET_F32OUT = 2, PVNET_GATHER = 4 };
  * ntaps*cin.  If relu_of is not NULL the result is multiplied by (relu_of[m][n] > 0) (the
  * backward of a ReLU whose output is relu_of).  flags: PVNET_RELU applies max(., 0);
  * PVNET_F32OUT writes float instead of bf16.  y may be a channel slice of a wider tensor.
- * 16- and 32-output 3x3x3 convolutions on grids of whole 16 x 4 x 4 bricks (x, y, z) stage each
- * brick and its halo in LDS once; PVNET_GATHER keeps them on the per-tap gathering kernel (the
+ * 3x3x3 convolutions with 16, 32, 48 or 64 outputs (or multiples of 48 or 64) on grids of whole
+ * 16 x 4 x 4 bricks (x, y, z) stage each brick and its halo in LDS once; PVNET_GATHER keeps them on the per-tap gathering kernel (the
  * same result bit for bit, for comparison). */
 typedef struct {
     pvnet_shape s;
@@ -58,9 +58,8 @@ int pvnet_conv_fwd(const pvnet_conv* c, void* stream);
 /* dw[n][k] = sum_m dy[m][n] * x[m + off(tap(k))][ci(k)]  for k < ntaps*cin (row length ntaps*cin)
  * db[n]    = sum_m dy[m][n]                               (if db is not NULL)
  * ws: FP32 scratch of at least pvnet_wgrad_ws_floats(...) floats for the split-K partials; the
- * sums are reduced in a fixed order, so results are deterministic.  16-output 3x3x3 layers with at
- * most 48 input channels on grids of whole 8 x 8 x 4 bricks reduce brick by brick with the input
- * staged in LDS; flags = PVNET_GATHER keeps them on the per-tap gathering kernel (the same sums in
+ * sums are reduced in a fixed order, so results are deterministic.  16- and 32-output 3x3x3 layers
+ * on grids of whole 8 x 8 x 4 bricks reduce brick by brick with the input staged in LDS; flags = PVNET_GATHER keeps them on the per-tap gathering kernel (the same sums in
  * another order, for comparison).  flags = 0 otherwise. */
 typedef struct {
     pvnet_shape s;

@@ -96,42 +96,52 @@ __global__ __launch_bounds__(256) void k_conv_fwd(pvnet_conv a, int M, int K) {
     const int m0 = xcd_tile(blockIdx.x, gridDim.x) * BM, n0 = blockIdx.y * BN;
     const int d = a.s.d, h = a.s.h, w = a.s.w;
 
-    // load roles: A rows r + 64 i, 8-wide k chunk q; B row bn (if < BN), same chunk
+    // load roles: A rows r + 64 i, 8-wide k chunk q; B row bn (if < BN), same chunk.  Per row, a
+    // mask of the taps whose neighbour lies in the box and the row's base pointer are set once;
+    // the chunk's (tap, channel) advance by 32 k-values per step.
     const int q = t & 3, r = t >> 2, bn = t >> 2;
-    int zz[NA], yy[NA], xx[NA];
-    bool ok[NA];
+    uint32_t tmask[NA];
+    const uint16_t* rowp[NA];
     #pragma unroll
     for (int i = 0; i < NA; i++) {
-        int m = m0 + r + 64 * i;
-        ok[i] = m < M;
-        int mm = ok[i] ? m : 0;
-        xx[i] = mm % w;
-        int tq = mm / w;
-        yy[i] = tq % h;
-        zz[i] = (tq / h) % d;
+        const int m = m0 + r + 64 * i;
+        const int mm = m < M ? m : 0;
+        const int xx = mm % w, tq = mm / w, yy = tq % h, zz = (tq / h) % d;
+        uint32_t mk = 0;
+        if (m < M) {
+            for (int tp = 0; tp < a.ntaps; tp++) {
+                int dz, dy, dx;
+                tap_offset(a.ntaps, tp, dz, dy, dx);
+                if ((unsigned)(zz + dz) < (unsigned)d && (unsigned)(yy + dy) < (unsigned)h && (unsigned)(xx + dx) < (unsigned)w)
+                    mk |= 1u << tp;
+            }
+        }
+        tmask[i] = mk;
+        rowp[i] = a.x + (int64_t)mm * a.ldx;
     }
     const bool brow = bn < BN && n0 + bn < a.cout;
+    const uint16_t* wp = a.w + (int64_t)(n0 + (brow ? bn : 0)) * a.kpad + q * 8;
     const int nsteps = a.kpad / BK;
     uint4 ra[NA], rb;
+    int tap = (q * 8) / a.cin, ci = q * 8 - tap * a.cin;  // chunk of step 0
 
     auto load = [&](int s) {
-        const int k = s * BK + q * 8;
-        const bool kin = k < K;
-        const int tap = kin ? k / a.cin : 0;
-        const int ci = k - tap * a.cin;
+        const bool kin = tap < a.ntaps;  // k = s * 32 + 8 q < K
         int dz, dy, dx;
-        tap_offset(a.ntaps, tap, dz, dy, dx);
-        const int64_t off = ((int64_t)dz * h + dy) * w + dx;
+        tap_offset(a.ntaps, kin ? tap : 0, dz, dy, dx);
+        const int64_t off = (((int64_t)dz * h + dy) * w + dx) * a.ldx + ci;
         #pragma unroll
         for (int i = 0; i < NA; i++) {
-            const int z2 = zz[i] + dz, y2 = yy[i] + dy, x2 = xx[i] + dx;
-            const bool in = ok[i] && kin && (unsigned)z2 < (unsigned)d && (unsigned)y2 < (unsigned)h &&
-                            (unsigned)x2 < (unsigned)w;
             ra[i] = make_uint4(0, 0, 0, 0);
-            if (in) ra[i] = *(const uint4*)(a.x + ((int64_t)(m0 + r + 64 * i) + off) * a.ldx + ci);
+            if (kin && ((tmask[i] >> tap) & 1u)) ra[i] = *(const uint4*)(rowp[i] + off);
         }
         rb = make_uint4(0, 0, 0, 0);
-        if (brow) rb = *(const uint4*)(a.w + (int64_t)(n0 + bn) * a.kpad + s * BK + q * 8);
+        if (brow) rb = *(const uint4*)(wp + s * BK);
+        ci += BK;  // the chunk of the next step
+        while (ci >= a.cin) {
+            ci -= a.cin;
+            tap++;
+        }
     };
 
     f32x4 acc[WM / 16][BN / 16];
@@ -181,7 +191,8 @@ __global__ __launch_bounds__(256) void k_conv_fwd(pvnet_conv a, int M, int K) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// 3x3x3 convolution with 16 or 32 output channels (NT column tiles) on grids of whole 16 x 4 x 4
+// 3x3x3 convolution with 16 NT output channels per block (NT = 1..4 column tiles; grid.y takes
+// further groups of 16 NT channels) on grids of whole 16 x 4 x 4
 // bricks: each block stages one brick of one box with its one-voxel halo (18 x 6 x 6 voxels, all
 // cin channels) in LDS once and runs the k-steps of k_conv_fwd over it, where k_conv_fwd gathers
 // every tap's rows from L2 again.  The k-steps, their operands and their order are those of
@@ -212,6 +223,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd_brick(pvnet_conv a, int K) {
     const int bz = bid % nbz;
     const int box = bid / nbz;
     const int x0 = bx * BRX, y0 = by * BRY, z0 = bz * BRZ;
+    const int n0 = blockIdx.y * 16 * NT;
     const int cch = a.cin >> 3;
     const int nsteps = a.kpad / BK;
     int* tab = reinterpret_cast<int*>(halo + HLV * cch);
@@ -249,7 +261,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd_brick(pvnet_conv a, int K) {
     const int vox = ((wv + 1) * HLY + 1) * HLX + c16 + 1 - BR_TAB_BIAS;
     const uint16_t* wrow[NT];
 #pragma unroll
-    for (int j = 0; j < NT; j++) wrow[j] = a.w + (int64_t)(16 * j + c16) * a.kpad + g * 8;
+    for (int j = 0; j < NT; j++) wrow[j] = a.w + (int64_t)(n0 + 16 * j + c16) * a.kpad + g * 8;
     int tv = tab[g];
     bf16x8 bfr[NT];
 #pragma unroll
@@ -289,7 +301,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd_brick(pvnet_conv a, int K) {
     const bool relu = a.flags & PVNET_RELU, f32out = a.flags & PVNET_F32OUT;
 #pragma unroll
     for (int j = 0; j < NT; j++) {
-        const int n = 16 * j + c16;
+        const int n = n0 + 16 * j + c16;
         const float bias = a.bias ? a.bias[n] : 0.f;
 #pragma unroll
         for (int i = 0; i < 4; i++)
@@ -428,9 +440,10 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(pvnet_wgrad a, int M, int K,
 }
 
 // ---------------------------------------------------------------------------------------------
-// weight gradient of a 16-output 3x3x3 convolution on grids of whole 8 x 8 x 4 bricks.  Block
-// `split` reduces a contiguous run of bricks; per brick it stages dy^T once (16 rows of 256 voxels)
-// and holds its eight MFMA A fragments (32 voxels each) in registers for every tap, then stages
+// weight gradient of a 16- or 32-output 3x3x3 convolution on grids of whole 8 x 8 x 4 bricks.
+// Block `split` reduces a contiguous run of bricks; per brick it stages dy^T once (16 or 32 rows of
+// 256 voxels) and holds its MFMA A fragments (8 per 16 rows, 32 voxels each) in registers for
+// every tap, so that each B fragment feeds one MFMA per 16 output channels, then stages
 // the input one 8-channel group at a time as three channel-major copies, one per x shift dx, of
 // the brick's rows with their z/y halo: X[dx][ci][zh][yh] = the 8 voxels x0 + dx + (0..7) of row
 // (z0 - 1 + zh, y0 - 1 + yh), one 16-byte LDS row.  A B fragment (32 voxels x 16 k-columns) is
@@ -472,25 +485,28 @@ __device__ __forceinline__ void store_cols(const uint4 (&r)[8], uint4* dst, int 
     }
 }
 
-// block (split, gy): bricks split * per .. ; channel groups gy * WB_NGB + (0 .. NGB - 1)
-template <int NGB>
+// block (split, gy): bricks split * per .. ; channel groups gy * WB_NGB + (0 .. NGB - 1); NR row
+// tiles of 16 output channels (cout = 16 NR)
+template <int NGB, int NR>
 __global__ __launch_bounds__(256) void k_conv_wgrad_brick(pvnet_wgrad a, int K, int nbricks, int per) {
     // X ends in WBX_ZERO zero rows: the lanes of the empty half of the last tap pair read there
     __shared__ uint4 X[WBX_ROWS + WBX_ZERO];
-    __shared__ uint4 DY[16 * WBD_P];
+    __shared__ uint4 DY[16 * NR * WBD_P];
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int g = lane >> 4, c16 = lane & 15;
     const int d = a.s.d, h = a.s.h, w = a.s.w;
     const int nbx = w / WBRX, nby = h / WBRY, nbz = d / WBRZ;
-    const int split = blockIdx.x, cg0 = blockIdx.y * WB_NGB;
+    const int split = blockIdx.x, cg0 = blockIdx.y * NGB;
     const int ngrp = min(NGB, a.cin / 8 - cg0);
     const int b0 = split * per, b1 = min(nbricks, b0 + per);
 
-    f32x4 acc[NGB][4];
+    f32x4 acc[NGB][4][NR];
 #pragma unroll
     for (int j = 0; j < NGB; j++)
 #pragma unroll
-        for (int i = 0; i < 4; i++) acc[j][i] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int rt = 0; rt < NR; rt++) acc[j][i][rt] = f32x4{0.f, 0.f, 0.f, 0.f};
 
     const int cl = c16 & 7;  // this lane's channel within the group; its tap is 2 p + (c16 >> 3)
     if (t < WBX_ZERO) X[WBX_ROWS + t] = make_uint4(0, 0, 0, 0);  // ordered by the first barrier
@@ -505,6 +521,15 @@ __global__ __launch_bounds__(256) void k_conv_wgrad_brick(pvnet_wgrad a, int K, 
         const int64_t box = bid / nbz;
         const int x0 = bx * WBRX, y0 = by * WBRY, z0 = bz * WBRZ;
 
+        // dy task (row rr, channel chunk nc): dy^T rows n = 8 nc .. 8 nc + 7 at voxels 8 rr ..
+        for (int u = t; u < 64 * NR; u += 256) {
+            const int rr = u / (2 * NR), nc = u % (2 * NR);
+            const int64_t row = ((box * d + z0 + (rr >> 3)) * h + y0 + (rr & 7)) * (int64_t)w + x0;
+            uint4 q[8];
+#pragma unroll
+            for (int i = 0; i < 8; i++) q[i] = *(const uint4*)(a.dy + (row + i) * a.lddy + nc * 8);
+            store_cols(q, DY + nc * 8 * WBD_P + rr, WBD_P);
+        }
 #pragma unroll
         for (int j = 0; j < NGB; j++) {
             if (j >= ngrp) break;  // block-uniform
@@ -524,20 +549,15 @@ __global__ __launch_bounds__(256) void k_conv_wgrad_brick(pvnet_wgrad a, int K, 
                     if (zy && (unsigned)x < (unsigned)w) r[i] = *(const uint4*)(a.x + (row + x) * a.ldx + cg * 8);
                 }
                 store_cols(r, X + dxi * WBX_CPY + zh * 10 + yh, WBX_CH);
-            } else if (j == 0 && t >= 192) {
-                // dy task (row r, channel chunk nc): dy^T rows n = 8 nc .. 8 nc + 7 at voxels 8 r ..
-                const int u = t - 192, rr = u >> 1, nc = u & 1;
-                const int64_t row = ((box * d + z0 + (rr >> 3)) * h + y0 + (rr & 7)) * (int64_t)w + x0;
-                uint4 q[8];
-#pragma unroll
-                for (int i = 0; i < 8; i++) q[i] = *(const uint4*)(a.dy + (row + i) * a.lddy + nc * 8);
-                store_cols(q, DY + nc * 8 * WBD_P + rr, WBD_P);
             }
             __syncthreads();
-            // A fragments: dy^T[n = c16][voxels 32 s + 8 g .. + 7] of brick row 4 s + g
-            bf16x8 af[8];
+            // A fragments: dy^T[n = 16 rt + c16][voxels 32 s + 8 g .. + 7] of brick row 4 s + g
+            bf16x8 af[NR][8];
 #pragma unroll
-            for (int s2 = 0; s2 < 8; s2++) af[s2] = *reinterpret_cast<const bf16x8*>(&DY[c16 * WBD_P + 4 * s2 + g]);
+            for (int rt = 0; rt < NR; rt++)
+#pragma unroll
+                for (int s2 = 0; s2 < 8; s2++)
+                    af[rt][s2] = *reinterpret_cast<const bf16x8*>(&DY[(16 * rt + c16) * WBD_P + 4 * s2 + g]);
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 const int p = wv + 4 * i;
@@ -550,15 +570,18 @@ __global__ __launch_bounds__(256) void k_conv_wgrad_brick(pvnet_wgrad a, int K, 
                 for (int s2 = 0; s2 < 8; s2++) {
                     const int r = 4 * s2 + g;  // brick row: z = r / 8, y = r % 8
                     const uint4 v = X[base + (r >> 3) * 10 + (r & 7)];
-                    acc[j][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[s2], *reinterpret_cast<const bf16x8*>(&v),
-                                                                        acc[j][i], 0, 0, 0);
+#pragma unroll
+                    for (int rt = 0; rt < NR; rt++)
+                        acc[j][i][rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                            af[rt][s2], *reinterpret_cast<const bf16x8*>(&v), acc[j][i][rt], 0, 0, 0);
                 }
             }
         }
         __syncthreads();  // X and DY are restaged for the next brick
     }
 
-    // D[4 g + e][c16]: n = 4 g + e, column c16 of tile p -> k = tap * cin + 8 cg + (c16 & 7)
+    // D[4 g + e][c16] of row tile rt: n = 16 rt + 4 g + e; column c16 of tile p -> k = tap * cin +
+    // 8 cg + (c16 & 7)
 #pragma unroll
     for (int j = 0; j < NGB; j++) {
         if (j >= ngrp) break;
@@ -570,7 +593,10 @@ __global__ __launch_bounds__(256) void k_conv_wgrad_brick(pvnet_wgrad a, int K, 
             if (tap >= 27) continue;
             const int k = tap * a.cin + (cg0 + j) * 8 + cl;
 #pragma unroll
-            for (int e = 0; e < 4; e++) a.ws[((int64_t)split * 16 + 4 * g + e) * K + k] = acc[j][i][e];
+            for (int rt = 0; rt < NR; rt++)
+#pragma unroll
+                for (int e = 0; e < 4; e++)
+                    a.ws[((int64_t)split * 16 * NR + 16 * rt + 4 * g + e) * K + k] = acc[j][i][rt][e];
         }
     }
 }
@@ -926,8 +952,8 @@ struct WgradPlan {
 // the brick kernel takes 16-output 3x3x3 layers with up to 48 input channels on grids of whole
 // bricks; PVNET_GATHER keeps them on k_conv_wgrad
 bool wgrad_brick_ok(const pvnet_shape& s, int ntaps, int cin, int cout) {
-    return ntaps == 27 && cout == 16 && cin % 8 == 0 && cin / 8 >= 1 && cin / 8 <= 6 && s.d % WBRZ == 0 &&
-           s.h % WBRY == 0 && s.w % WBRX == 0;
+    return ntaps == 27 && (cout == 16 || cout == 32) && cin % 8 == 0 && s.d % WBRZ == 0 && s.h % WBRY == 0 &&
+           s.w % WBRX == 0;
 }
 
 int wgrad_bn(int cout) { return cout <= 16 ? 16 : (cout <= 32 || cout % 64 != 0) ? 32 : 64; }
@@ -954,7 +980,7 @@ WgradPlan plan_wgrad(const pvnet_shape& s, int ntaps, int cin, int cout, int fla
         p.nsplit = (p.nbricks + p.per - 1) / p.per;
         p.ws_main = (int64_t)p.nsplit * cout * K;
     }
-    p.cchunk = 8192;
+    p.cchunk = (int)std::max<int64_t>(256, ((M + 1023) / 1024 + 255) & ~(int64_t)255);  // ~1024 blocks
     p.cblk = (int)((M + p.cchunk - 1) / p.cchunk);
     p.ws_total = p.ws_main + (int64_t)p.cblk * cout;
     return p;
@@ -991,13 +1017,17 @@ int pvnet_conv_fwd(const pvnet_conv* c, void* stream) {
     // grid still has >= 2048 blocks (the K loop is long, so the device needs many blocks in flight)
     const int mt = blocks_for(M, 128);
     const size_t halo_bytes = (size_t)HLV * c->cin * 2 + (size_t)(c->kpad / BK) * 4 * sizeof(int);
-    if ((c->cout == 16 || c->cout == 32) && c->ntaps == 27 && !(c->flags & PVNET_GATHER) && c->s.d % BRZ == 0 &&
-        c->s.h % BRY == 0 && c->s.w % BRX == 0 && halo_bytes <= 64 * 1024 && M >= (int64_t)256 * 2048) {
-        const dim3 gb((unsigned)(M / (BRX * BRY * BRZ)));
-        if (c->cout == 16)
-            hipLaunchKernelGGL(k_conv_fwd_brick<1>, gb, dim3(256), halo_bytes, st, *c, K);
-        else
-            hipLaunchKernelGGL(k_conv_fwd_brick<2>, gb, dim3(256), halo_bytes, st, *c, K);
+    // brick kernel: 16 NT channels per block, NT = cout / 16 up to 4 (48 and 96: NT = 3)
+    const int bnt = c->cout % 16 ? 0 : c->cout <= 64 ? c->cout / 16 : c->cout % 48 == 0 ? 3 : c->cout % 64 == 0 ? 4 : 0;
+    if (bnt && c->ntaps == 27 && !(c->flags & PVNET_GATHER) && c->s.d % BRZ == 0 && c->s.h % BRY == 0 &&
+        c->s.w % BRX == 0 && halo_bytes <= 64 * 1024 && M >= (int64_t)256 * 2048) {
+        const dim3 gb((unsigned)(M / (BRX * BRY * BRZ)), c->cout / (16 * bnt));
+        switch (bnt) {
+            case 1: hipLaunchKernelGGL(k_conv_fwd_brick<1>, gb, dim3(256), halo_bytes, st, *c, K); break;
+            case 2: hipLaunchKernelGGL(k_conv_fwd_brick<2>, gb, dim3(256), halo_bytes, st, *c, K); break;
+            case 3: hipLaunchKernelGGL(k_conv_fwd_brick<3>, gb, dim3(256), halo_bytes, st, *c, K); break;
+            default: hipLaunchKernelGGL(k_conv_fwd_brick<4>, gb, dim3(256), halo_bytes, st, *c, K); break;
+        }
     } else if (c->cout <= 16 && M >= (int64_t)256 * 2048) {
         hipLaunchKernelGGL((k_conv_fwd<16, 64>), dim3(blocks_for(M, 256), 1), dim3(256), 0, st, *c, (int)M, K);
     } else if (c->cout <= 16) {
@@ -1036,19 +1066,20 @@ int pvnet_conv_wgrad(const pvnet_wgrad* a, void* stream) {
     if (p.brick) {
         // one round of resident blocks along the voxels (at most the plan's WB_SPLITS partial rows)
         const int ng = a->cin / 8;
-        const void* kern = ng == 1 ? (const void*)k_conv_wgrad_brick<1> : (const void*)k_conv_wgrad_brick<WB_NGB>;
+        void (*kern)(pvnet_wgrad, int, int, int) =
+            a->cout == 32 ? (ng == 1 ? k_conv_wgrad_brick<1, 2> : k_conv_wgrad_brick<WB_NGB, 2>)
+                          : (ng == 1 ? k_conv_wgrad_brick<1, 1> : k_conv_wgrad_brick<WB_NGB, 1>);
         int dev = 0, cus = 0, occ = 0;
         NET_HIP(hipGetDevice(&dev));
         NET_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
         NET_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, 256, 0));
-        const int want = std::max(1, std::min({p.nbricks, WB_SPLITS, std::max(1, cus * occ)}));
+        const int gy = ng == 1 ? 1 : (ng + WB_NGB - 1) / WB_NGB;
+        // one round of resident blocks (at most the plan's WB_SPLITS partial rows)
+        const int want = std::max(1, std::min({p.nbricks, WB_SPLITS, std::max(1, cus * occ / gy)}));
         const int per = (p.nbricks + want - 1) / want;
         nsplit = (p.nbricks + per - 1) / per;
-        const dim3 gb(nsplit, (ng + WB_NGB - 1) / WB_NGB);
-        if (ng == 1)
-            hipLaunchKernelGGL(k_conv_wgrad_brick<1>, gb, dim3(256), 0, st, *a, K, p.nbricks, per);
-        else
-            hipLaunchKernelGGL(k_conv_wgrad_brick<WB_NGB>, gb, dim3(256), 0, st, *a, K, p.nbricks, per);
+        const dim3 gb(nsplit, gy);
+        hipLaunchKernelGGL(kern, gb, dim3(256), 0, st, *a, K, p.nbricks, per);
     } else if (p.bn == 16)
         hipLaunchKernelGGL(k_conv_wgrad<16>, grid, dim3(256), 0, st, *a, (int)M, K, p.mchunk);
     else if (p.bn == 32)

@@ -170,10 +170,11 @@ def test_conv_fwd_large_grid_tiles(T):
 @pytest.mark.parametrize("cin,cout,flags,ldx,xoff,ldy,yoff,mask", [
     (8, 16, 1, 8, 0, 16, 0, False), (16, 16, 1, 24, 8, 48, 32, False), (32, 16, 2, 32, 0, 16, 0, False),
     (48, 16, 0, 48, 0, 16, 0, True), (16, 32, 1, 16, 0, 64, 32, False), (32, 32, 0, 40, 8, 32, 0, True),
-    (48, 32, 3, 48, 0, 32, 0, False)])
+    (48, 32, 3, 48, 0, 32, 0, False), (16, 48, 0, 16, 0, 48, 0, True), (32, 64, 1, 32, 0, 64, 0, False),
+    (16, 96, 0, 24, 8, 96, 0, True)])
 def test_conv_fwd_brick_equals_gather(T, cin, cout, flags, ldx, xoff, ldy, yoff, mask):
-    """16- and 32-output 3x3x3 convolutions on whole 16 x 4 x 4 bricks stage each brick with its
-    halo in LDS; the result must equal the per-tap gathering kernel's (PVNET_GATHER) bit for bit,
+    """3x3x3 convolutions with 16 to 96 outputs on whole 16 x 4 x 4 bricks stage each brick with
+    its halo in LDS; the result must equal the per-tap gathering kernel's (PVNET_GATHER) bit for bit,
     over two boxes (zero padding at every box face), channel slices of wider tensors, ReLU, FP32
     output and the ReLU-backward mask.  Columns outside the output slice stay untouched."""
     from pv_dec_original_amd import _native as N
@@ -205,18 +206,18 @@ def test_conv_fwd_brick_equals_gather(T, cin, cout, flags, ldx, xoff, ldy, yoff,
     assert a[:, yoff:yoff + cout].abs().max().item() > 0
 
 
-@pytest.mark.parametrize("shape,cin,ldx,xoff", [
-    ((2, 8, 16, 16), 8, 8, 0), ((1, 12, 16, 24), 16, 24, 8), ((3, 4, 8, 8), 24, 24, 0),
-    ((2, 8, 8, 16), 48, 56, 8)])
-def test_wgrad_brick_against_oracle_and_gather(T, shape, cin, ldx, xoff):
-    """16-output 3x3x3 weight gradients on grids of whole 8 x 8 x 4 bricks reduce brick by brick
-    with the input staged in LDS: against the FP64 oracle (FP32 accumulation, 1e-5 of the scale)
-    and against the per-tap gathering kernel (PVNET_GATHER, the same sums in another order), over
-    several boxes (zero padding at every face), one to six 8-channel groups, and an input slice of
-    a wider tensor; two runs give the same bits."""
+@pytest.mark.parametrize("shape,cin,cout,ldx,xoff", [
+    ((2, 8, 16, 16), 8, 16, 8, 0), ((1, 12, 16, 24), 16, 16, 24, 8), ((3, 4, 8, 8), 24, 16, 24, 0),
+    ((2, 8, 8, 16), 48, 16, 56, 8), ((2, 8, 16, 8), 8, 32, 8, 0), ((1, 8, 8, 16), 32, 32, 48, 16),
+    ((2, 4, 8, 8), 96, 32, 96, 0)])
+def test_wgrad_brick_against_oracle_and_gather(T, shape, cin, cout, ldx, xoff):
+    """16- and 32-output 3x3x3 weight gradients on grids of whole 8 x 8 x 4 bricks reduce brick by
+    brick with the input staged in LDS: against the FP64 oracle (FP32 accumulation, 1e-5 of the
+    scale) and against the per-tap gathering kernel (PVNET_GATHER, the same sums in another order),
+    over several boxes (zero padding at every face), one to twelve 8-channel groups, and an input
+    slice of a wider tensor; two runs give the same bits."""
     from pv_dec_original_amd import _native as N
     NL = N.net()
-    cout = 16
     M = O.nvox(shape)
     xfull = np.zeros((M, ldx))
     xfull[:, xoff:xoff + cin] = _rbf((M, cin))
