@@ -90,6 +90,11 @@ int pvnet_pool2_fwd(const pvnet_shape* s, int32_t c, const uint16_t* x, int32_t 
                     int32_t ldy, void* stream);
 int pvnet_pool2_bwd(const pvnet_shape* s, int32_t c, const uint16_t* dy, int32_t lddy, uint16_t* dx,
                     int32_t lddx, int32_t accumulate, void* stream);
+/* The accumulating backward followed by the backward of a ReLU whose output is relu_of, in one
+ * pass: dx = (relu_of > 0) ? dx + dy/8 : 0 (the same values as pvnet_pool2_bwd with accumulate = 1
+ * and then pvnet_relu_bwd on dx). */
+int pvnet_pool2_bwd_relu(const pvnet_shape* s, int32_t c, const uint16_t* dy, int32_t lddy, uint16_t* dx,
+                         int32_t lddx, const uint16_t* relu_of, int32_t ldr, void* stream);
 
 /* nearest 2x upsampling: y (shape s) from x (shape s/2); backward sums the 8 children. */
 int pvnet_up2_fwd(const pvnet_shape* s, int32_t c, const uint16_t* x, int32_t ldx, uint16_t* y,

@@ -129,6 +129,7 @@ PVNET_API = [
     ("pvnet_pack_weights", C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32, _vp, _vp, _vp]),
     ("pvnet_pool2_fwd", C.c_int, [_sp, C.c_int32, _vp, C.c_int32, _vp, C.c_int32, _vp]),
     ("pvnet_pool2_bwd", C.c_int, [_sp, C.c_int32, _vp, C.c_int32, _vp, C.c_int32, C.c_int32, _vp]),
+    ("pvnet_pool2_bwd_relu", C.c_int, [_sp, C.c_int32, _vp, C.c_int32, _vp, C.c_int32, _vp, C.c_int32, _vp]),
     ("pvnet_up2_fwd", C.c_int, [_sp, C.c_int32, _vp, C.c_int32, _vp, C.c_int32, _vp]),
     ("pvnet_up2_bwd", C.c_int, [_sp, C.c_int32, _vp, C.c_int32, _vp, C.c_int32, _vp]),
     ("pvnet_relu_bwd", C.c_int, [C.c_int64, C.c_int32, _vp, C.c_int32, _vp, C.c_int32, _vp]),

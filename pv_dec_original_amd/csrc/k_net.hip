@@ -485,10 +485,12 @@ __device__ __forceinline__ void store_cols(const uint4 (&r)[8], uint4* dst, int 
     }
 }
 
-// block (split, gy): bricks split * per .. ; channel groups gy * WB_NGB + (0 .. NGB - 1); NR row
-// tiles of 16 output channels (cout = 16 NR)
+// block (split, gy): bricks split * per .. ; channel groups gy * NGB + (0 .. NGB - 1); NR row tiles
+// of 16 output channels (cout = 16 NR).  The grid is one-dimensional, ngy blocks per split: the
+// blocks of one split take consecutive slots of one XCD, so that they walk the same bricks at the
+// same time through one L2 (each reads its own channels of the same input rows).
 template <int NGB, int NR>
-__global__ __launch_bounds__(256) void k_conv_wgrad_brick(pvnet_wgrad a, int K, int nbricks, int per) {
+__global__ __launch_bounds__(256) void k_conv_wgrad_brick(pvnet_wgrad a, int K, int nbricks, int per, int ngy) {
     // X ends in WBX_ZERO zero rows: the lanes of the empty half of the last tap pair read there
     __shared__ uint4 X[WBX_ROWS + WBX_ZERO];
     __shared__ uint4 DY[16 * NR * WBD_P];
@@ -496,7 +498,8 @@ __global__ __launch_bounds__(256) void k_conv_wgrad_brick(pvnet_wgrad a, int K, 
     const int g = lane >> 4, c16 = lane & 15;
     const int d = a.s.d, h = a.s.h, w = a.s.w;
     const int nbx = w / WBRX, nby = h / WBRY, nbz = d / WBRZ;
-    const int split = blockIdx.x, cg0 = blockIdx.y * NGB;
+    const int slot = xcd_tile(blockIdx.x, gridDim.x);
+    const int split = slot / ngy, cg0 = (slot % ngy) * NGB;
     const int ngrp = min(NGB, a.cin / 8 - cg0);
     const int b0 = split * per, b1 = min(nbricks, b0 + per);
 
@@ -660,12 +663,15 @@ __device__ __forceinline__ int64_t fine_index(const pvnet_shape& s, int64_t b, i
     return ((b * s.d + z) * s.h + y) * (int64_t)s.w + x;
 }
 
+// coarse voxel and 8-channel chunk of thread i (32-bit arithmetic: the host keeps the coarse
+// voxel count times the chunk count below 2^31)
 __device__ __forceinline__ bool coarse_of(const pvnet_shape& s, int c8n, int64_t i, Coarse& o, int& ch) {
-    const int d2 = s.d / 2, h2 = s.h / 2, w2 = s.w / 2;
+    const uint32_t d2 = s.d / 2, h2 = s.h / 2, w2 = s.w / 2;
     const int64_t n = (int64_t)s.nb * d2 * h2 * w2 * c8n;
     if (i >= n) return false;
-    ch = (int)(i % c8n) * 8;
-    int64_t v = i / c8n;
+    uint32_t v = (uint32_t)i;
+    ch = (int)(v % (uint32_t)c8n) * 8;
+    v /= (uint32_t)c8n;
     o.x = (int)(v % w2);
     v /= w2;
     o.y = (int)(v % h2);
@@ -696,7 +702,10 @@ __global__ void k_pool2_fwd(pvnet_shape s, int c, const uint16_t* x, int ldx, ui
     *(uint4*)(y + coarse_index(s, o) * ldy + ch) = out;
 }
 
-__global__ void k_pool2_bwd(pvnet_shape s, int c, const uint16_t* dy, int lddy, uint16_t* dx, int lddx, int accum) {
+// relu_of (optional): the children's results are multiplied by (relu_of > 0), the backward of a
+// ReLU whose output is relu_of, in the same pass
+__global__ void k_pool2_bwd(pvnet_shape s, int c, const uint16_t* dy, int lddy, uint16_t* dx, int lddx, int accum,
+                            const uint16_t* relu_of, int ldr) {
     Coarse o;
     int ch;
     if (!coarse_of(s, c / 8, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, o, ch)) return;
@@ -715,6 +724,12 @@ __global__ void k_pool2_bwd(pvnet_shape s, int c, const uint16_t* dy, int lddy, 
             for (int j = 0; j < 8; j++) po[j] = f2bf(bf2f(pold[j]) + g[j]);
         } else {
             for (int j = 0; j < 8; j++) po[j] = f2bf(g[j]);
+        }
+        if (relu_of) {
+            const uint4 av = *(const uint4*)(relu_of + f * ldr + ch);
+            const uint16_t* pa = (const uint16_t*)&av;
+            for (int j = 0; j < 8; j++)
+                if (!bf_pos(pa[j])) po[j] = 0;
         }
         *dst = out;
     }
@@ -748,11 +763,11 @@ __global__ void k_up2_bwd(pvnet_shape s, int c, const uint16_t* dy, int lddy, ui
     *(uint4*)(dx + coarse_index(s, o) * lddx + ch) = out;
 }
 
-__global__ void k_relu_bwd(int64_t m, int c8n, const uint16_t* a, int lda, uint16_t* dx, int lddx) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m * c8n) return;
-    const int64_t v = i / c8n;
-    const int ch = (int)(i % c8n) * 8;
+// grid.y: the 8-channel chunk
+__global__ void k_relu_bwd(int64_t m, const uint16_t* a, int lda, uint16_t* dx, int lddx) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= m) return;
+    const int ch = blockIdx.y * 8;
     const uint4 av = *(const uint4*)(a + v * lda + ch);
     uint4* dp = (uint4*)(dx + v * lddx + ch);
     uint4 dv = *dp;
@@ -767,8 +782,12 @@ __global__ void k_relu_bwd(int64_t m, int c8n, const uint16_t* a, int lda, uint1
 // starts from g_in = 0 on solid voxels and "infinite" elsewhere.  One thread per line.
 constexpr float EDT_INF = 1e30f;
 
-// One thread per line; the block's lines are staged in LDS (row pitch n + 1: the threads' reads
-// of one column fall in distinct banks).
+// One thread per line; the block's lines are staged in LDS (row pitch n + 1: the threads' reads of
+// one column fall in distinct banks).  The first pass (binary input: 0 on solid voxels) is the
+// squared distance to the nearest solid voxel of the line, from one sweep each way.  The later
+// passes give each voxel min_j (f(j) + (i - j)^2) by scanning outward from i and stopping at the
+// first r with r^2 >= the best value so far: f >= 0, so no farther j can lower it.  Every term is
+// an exact integer in FP32 (or EDT_INF), so the result is that of the full scan bit for bit.
 __global__ __launch_bounds__(64) void k_edt_pass(pvnet_shape s, int axis, const uint8_t* code, float* g) {
     extern __shared__ float sh[];
     const int64_t L = (int64_t)s.nb * s.d * s.h * s.w;
@@ -790,13 +809,30 @@ __global__ __launch_bounds__(64) void k_edt_pass(pvnet_shape s, int axis, const 
         base = b * (int64_t)s.d * s.h * s.w + xy;
         stride = (int64_t)s.w * s.h;
     }
-    for (int j = 0; j < n; j++)
-        in[j] = axis == 0 ? (code[base + j * stride] == 0 ? 0.f : EDT_INF) : g[base + j * stride];
-    for (int i = 0; i < n; i++) {
-        float best = EDT_INF;
+    if (axis == 0) {
+        // in[j] = distance to the nearest solid voxel at or left of j (n + n if none), then the
+        // sweep back takes the right side
+        int last = -2 * n;
         for (int j = 0; j < n; j++) {
-            const float dj = (float)(i - j);
-            best = fminf(best, in[j] + dj * dj);
+            if (code[base + j] == 0) last = j;
+            in[j] = (float)(j - last);
+        }
+        int next = 3 * n;
+        for (int j = n - 1; j >= 0; j--) {
+            if (code[base + j] == 0) next = j;
+            const int r = min((int)in[j], next - j);
+            g[base + j] = r >= n ? EDT_INF : (float)(r * r);
+        }
+        return;
+    }
+    for (int j = 0; j < n; j++) in[j] = g[base + j * stride];
+    for (int i = 0; i < n; i++) {
+        float best = in[i];
+        for (int r = 1; r < n; r++) {
+            const float rr = (float)(r * r);
+            if (rr >= best) break;
+            if (i - r >= 0) best = fminf(best, in[i - r] + rr);
+            if (i + r < n) best = fminf(best, in[i + r] + rr);
         }
         g[base + i * stride] = axis == 2 ? fminf(sqrtf(best), 255.f) : best;
     }
@@ -992,6 +1028,7 @@ int pool_args(const pvnet_shape* s, int c, const void* a, int lda, const void* b
     if (!shape_ok(s) || s->d % 2 || s->h % 2 || s->w % 2) return fail(1, std::string(who) + ": sizes must be even");
     if (c <= 0 || c % 8 || lda < c || ldb < c || lda % 8 || ldb % 8) return fail(1, std::string(who) + ": bad channels");
     if (!a || !b || !al16(a) || !al16(b)) return fail(1, std::string(who) + ": null or misaligned pointer");
+    if (voxels(*s) / 8 * (c / 8) >= ((int64_t)1 << 31)) return fail(1, std::string(who) + ": too many voxels");
     return 0;
 }
 
@@ -1066,7 +1103,7 @@ int pvnet_conv_wgrad(const pvnet_wgrad* a, void* stream) {
     if (p.brick) {
         // one round of resident blocks along the voxels (at most the plan's WB_SPLITS partial rows)
         const int ng = a->cin / 8;
-        void (*kern)(pvnet_wgrad, int, int, int) =
+        void (*kern)(pvnet_wgrad, int, int, int, int) =
             a->cout == 32 ? (ng == 1 ? k_conv_wgrad_brick<1, 2> : k_conv_wgrad_brick<WB_NGB, 2>)
                           : (ng == 1 ? k_conv_wgrad_brick<1, 1> : k_conv_wgrad_brick<WB_NGB, 1>);
         int dev = 0, cus = 0, occ = 0;
@@ -1078,8 +1115,7 @@ int pvnet_conv_wgrad(const pvnet_wgrad* a, void* stream) {
         const int want = std::max(1, std::min({p.nbricks, WB_SPLITS, std::max(1, cus * occ / gy)}));
         const int per = (p.nbricks + want - 1) / want;
         nsplit = (p.nbricks + per - 1) / per;
-        const dim3 gb(nsplit, gy);
-        hipLaunchKernelGGL(kern, gb, dim3(256), 0, st, *a, K, p.nbricks, per);
+        hipLaunchKernelGGL(kern, dim3(nsplit * gy), dim3(256), 0, st, *a, K, p.nbricks, per, gy);
     } else if (p.bn == 16)
         hipLaunchKernelGGL(k_conv_wgrad<16>, grid, dim3(256), 0, st, *a, (int)M, K, p.mchunk);
     else if (p.bn == 32)
@@ -1129,7 +1165,18 @@ int pvnet_pool2_bwd(const pvnet_shape* s, int32_t c, const uint16_t* dy, int32_t
     if (int rc = pool_args(s, c, dy, lddy, dx, lddx, "pvnet_pool2_bwd")) return rc;
     const int64_t n = voxels(*s) / 8 * (c / 8);
     hipLaunchKernelGGL(k_pool2_bwd, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream, *s, c, dy, lddy, dx,
-                       lddx, accumulate ? 1 : 0);
+                       lddx, accumulate ? 1 : 0, (const uint16_t*)nullptr, 0);
+    NET_LAUNCH();
+    return 0;
+}
+
+int pvnet_pool2_bwd_relu(const pvnet_shape* s, int32_t c, const uint16_t* dy, int32_t lddy, uint16_t* dx,
+                         int32_t lddx, const uint16_t* relu_of, int32_t ldr, void* stream) {
+    if (int rc = pool_args(s, c, dy, lddy, dx, lddx, "pvnet_pool2_bwd_relu")) return rc;
+    if (!relu_of || !al16(relu_of) || ldr < c || ldr % 8) return fail(1, "pvnet_pool2_bwd_relu: bad relu_of");
+    const int64_t n = voxels(*s) / 8 * (c / 8);
+    hipLaunchKernelGGL(k_pool2_bwd, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream, *s, c, dy, lddy, dx,
+                       lddx, 1, relu_of, ldr);
     NET_LAUNCH();
     return 0;
 }
@@ -1155,8 +1202,7 @@ int pvnet_up2_bwd(const pvnet_shape* s, int32_t c, const uint16_t* dy, int32_t l
 int pvnet_relu_bwd(int64_t m, int32_t c, const uint16_t* a, int32_t lda, uint16_t* dx, int32_t lddx, void* stream) {
     if (m <= 0 || c <= 0 || c % 8 || lda < c || lddx < c || lda % 8 || lddx % 8 || !a || !dx || !al16(a) || !al16(dx))
         return fail(1, "pvnet_relu_bwd: bad argument");
-    const int64_t n = m * (c / 8);
-    hipLaunchKernelGGL(k_relu_bwd, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream, m, c / 8, a, lda, dx, lddx);
+    hipLaunchKernelGGL(k_relu_bwd, dim3(blocks_for(m, 256), c / 8), dim3(256), 0, (hipStream_t)stream, m, a, lda, dx, lddx);
     NET_LAUNCH();
     return 0;
 }

@@ -313,10 +313,9 @@ class UNet:
         for k in range(L - 2, -1, -1):
             ccat = lv[k + 1] + lv[k]
             ea, eb = by[f"e{k}a"], by[f"e{k}b"]
-            N.net_check(NL.pvnet_pool2_bwd(C.byref(S[k]), lv[k], _ptr(b[f"dP{k + 1}"]), lv[k],
-                                           _ptr(b[f"dCAT{k}"], lv[k + 1]), ccat, 1, self._st()))
-            N.net_check(NL.pvnet_relu_bwd(b["M"][k], lv[k], _ptr(b[f"CAT{k}"], lv[k + 1]), ccat,
-                                          _ptr(b[f"dCAT{k}"], lv[k + 1]), ccat, self._st()))
+            N.net_check(NL.pvnet_pool2_bwd_relu(C.byref(S[k]), lv[k], _ptr(b[f"dP{k + 1}"]), lv[k],
+                                                _ptr(b[f"dCAT{k}"], lv[k + 1]), ccat, _ptr(b[f"CAT{k}"], lv[k + 1]),
+                                                ccat, self._st()))
             self._wgrad(eb, S[k], b[f"A{k}"], lv[k], b[f"dCAT{k}"], ccat, dyoff=lv[k + 1])
             self._dgrad(eb, S[k], b[f"dCAT{k}"], ccat, b[f"dA{k}"], lv[k], relu_of=b[f"A{k}"], ldr=lv[k],
                         dyoff=lv[k + 1])

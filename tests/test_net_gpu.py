@@ -299,6 +299,16 @@ def test_pool_up_relu_features_loss_adam(T):
     N.net_check(NL.pvnet_relu_bwd(M, c, _p(ad), c, _p(gd), c, st))
     T.cuda.synchronize()
     assert np.array_equal(_np(gd), g * (a > 0))
+    # the fused pooling backward with the ReLU mask equals the two calls, on channel slices of rows
+    # of 3c channels
+    wide_old, wide_a = _rbf((M, 3 * c)), _rbf((M, 3 * c))
+    two, one, wad = _dev(T, wide_old), _dev(T, wide_old), _dev(T, wide_a)
+    N.net_check(NL.pvnet_pool2_bwd(C.byref(S), c, _p(dyc), c, _p(two, c), 3 * c, 1, st))
+    N.net_check(NL.pvnet_relu_bwd(M, c, _p(wad, c), 3 * c, _p(two, c), 3 * c, st))
+    N.net_check(NL.pvnet_pool2_bwd_relu(C.byref(S), c, _p(dyc), c, _p(one, c), 3 * c, _p(wad, c), 3 * c, st))
+    T.cuda.synchronize()
+    assert T.equal(one, two)
+    assert T.equal(one[:, :c], _dev(T, wide_old)[:, :c]) and T.equal(one[:, 2 * c:], _dev(T, wide_old)[:, 2 * c:])
     # wall distance and features
     code = RNG.integers(0, 4, shape).astype(np.uint8)
     code[RNG.random(shape) < 0.5] = 1
