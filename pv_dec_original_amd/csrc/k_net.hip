@@ -201,20 +201,26 @@ __global__ __launch_bounds__(256) void k_conv_fwd(pvnet_conv a, int M, int K) {
 // (halo[chunk][voxel], a chunk = 8 channels, 16 bytes), so the 16 lanes that read one A fragment
 // row group address 16 consecutive 16-byte rows: one pass over the 64 banks.
 
-constexpr int BRX = 16, BRY = 4, BRZ = 4;
-constexpr int HLX = BRX + 2, HLY = BRY + 2, HLZ = BRZ + 2, HLV = HLX * HLY * HLZ;
+// bricks of BRX x BY x BRZ voxels (BY = 4 or 8, one wave per 4 rows of a plane), halo HLX x (BY + 2)
+// x HLZ
+constexpr int BRX = 16, BRZ = 4;
+constexpr int HLX = BRX + 2, HLZ = BRZ + 2;
+template <int BY>
+constexpr int hl_vox() { return HLX * (BY + 2) * HLZ; }
 
 // step table: entry 4 s + g = the halo offset of k-values s * 32 + 8 g .. + 7 relative to the
 // output voxel (chunk * HLV + tap shift), plus BR_TAB_BIAS; 0 where they lie beyond K
 constexpr int BR_TAB_BIAS = 1 << 20;
 
-template <int NT>
-__global__ __launch_bounds__(256) void k_conv_fwd_brick(pvnet_conv a, int K) {
+template <int NT, int BY>
+__global__ __launch_bounds__(64 * BY) void k_conv_fwd_brick(pvnet_conv a, int K) {
+    constexpr int HLY = BY + 2, HLV = hl_vox<BY>(), NTH = 64 * BY;
     extern __shared__ uint4 halo[];  // [cin / 8 chunks][HLV voxels], then the step table
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int zz = wv % BRZ, yq = wv / BRZ;  // this wave: plane zz, rows 4 yq .. 4 yq + 3
     const int g = lane >> 4, c16 = lane & 15;
     const int d = a.s.d, h = a.s.h, w = a.s.w;
-    const int nbx = w / BRX, nby = h / BRY, nbz = d / BRZ;
+    const int nbx = w / BRX, nby = h / BY, nbz = d / BRZ;
     int bid = xcd_tile(blockIdx.x, gridDim.x);
     const int bx = bid % nbx;
     bid /= nbx;
@@ -222,12 +228,12 @@ __global__ __launch_bounds__(256) void k_conv_fwd_brick(pvnet_conv a, int K) {
     bid /= nby;
     const int bz = bid % nbz;
     const int box = bid / nbz;
-    const int x0 = bx * BRX, y0 = by * BRY, z0 = bz * BRZ;
+    const int x0 = bx * BRX, y0 = by * BY, z0 = bz * BRZ;
     const int n0 = blockIdx.y * 16 * NT;
     const int cch = a.cin >> 3;
     const int nsteps = a.kpad / BK;
     int* tab = reinterpret_cast<int*>(halo + HLV * cch);
-    for (int i = t; i < 4 * nsteps; i += 256) {
+    for (int i = t; i < 4 * nsteps; i += NTH) {
         const int k = (i >> 2) * BK + (i & 3) * 8;
         int v = 0;
         if (k < K) {
@@ -238,7 +244,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd_brick(pvnet_conv a, int K) {
         }
         tab[i] = v;
     }
-    for (int i = t; i < HLV * cch; i += 256) {
+    for (int i = t; i < HLV * cch; i += NTH) {
         const int ch = i / HLV, v = i - ch * HLV;
         const int x = x0 - 1 + v % HLX, y = y0 - 1 + (v / HLX) % HLY, z = z0 - 1 + v / (HLX * HLY);
         uint4 val = make_uint4(0, 0, 0, 0);
@@ -258,7 +264,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd_brick(pvnet_conv a, int K) {
     // halo voxel of (c16, tile 0, wv); tile i adds i rows.  Only the last step can reach beyond K
     // (kpad - K < 32), so the others run without a check, each prefetching the next step's table
     // entry and weight fragments.
-    const int vox = ((wv + 1) * HLY + 1) * HLX + c16 + 1 - BR_TAB_BIAS;
+    const int vox = ((zz + 1) * HLY + 4 * yq + 1) * HLX + c16 + 1 - BR_TAB_BIAS;
     const uint16_t* wrow[NT];
 #pragma unroll
     for (int j = 0; j < NT; j++) wrow[j] = a.w + (int64_t)(n0 + 16 * j + c16) * a.kpad + g * 8;
@@ -307,8 +313,8 @@ __global__ __launch_bounds__(256) void k_conv_fwd_brick(pvnet_conv a, int K) {
         for (int i = 0; i < 4; i++)
 #pragma unroll
             for (int e = 0; e < 4; e++) {
-                // MFMA row 4 g + e: voxel x0 + 4 g + e of row y0 + i in plane z0 + wv
-                const int64_t m = (((int64_t)box * d + z0 + wv) * h + y0 + i) * w + x0 + 4 * g + e;
+                // MFMA row 4 g + e: voxel x0 + 4 g + e of row y0 + 4 yq + i in plane z0 + zz
+                const int64_t m = (((int64_t)box * d + z0 + zz) * h + y0 + 4 * yq + i) * w + x0 + 4 * g + e;
                 float v = acc[i][j][e] + bias;
                 if (relu) v = v < 0.f ? 0.f : v;
                 if (a.relu_of && !bf_pos(a.relu_of[m * a.ldr + n])) v = 0.f;
@@ -1053,17 +1059,33 @@ int pvnet_conv_fwd(const pvnet_conv* c, void* stream) {
     // tiles: 256-voxel blocks for narrow outputs on large grids; 64-channel tiles only when the
     // grid still has >= 2048 blocks (the K loop is long, so the device needs many blocks in flight)
     const int mt = blocks_for(M, 128);
-    const size_t halo_bytes = (size_t)HLV * c->cin * 2 + (size_t)(c->kpad / BK) * 4 * sizeof(int);
-    // brick kernel: 16 NT channels per block, NT = cout / 16 up to 4 (48 and 96: NT = 3)
+    // brick kernel: 16 NT channels per block, NT = cout / 16 up to 4 (48 and 96: NT = 3); bricks of
+    // 16 x 8 x 4 voxels where their halo fits 64 KB of LDS, else 16 x 4 x 4
+    const size_t tab_bytes = (size_t)(c->kpad / BK) * 4 * sizeof(int);
+    const size_t halo8 = (size_t)hl_vox<8>() * c->cin * 2 + tab_bytes, halo4 = (size_t)hl_vox<4>() * c->cin * 2 + tab_bytes;
     const int bnt = c->cout % 16 ? 0 : c->cout <= 64 ? c->cout / 16 : c->cout % 48 == 0 ? 3 : c->cout % 64 == 0 ? 4 : 0;
-    if (bnt && c->ntaps == 27 && !(c->flags & PVNET_GATHER) && c->s.d % BRZ == 0 && c->s.h % BRY == 0 &&
-        c->s.w % BRX == 0 && halo_bytes <= 64 * 1024 && M >= (int64_t)256 * 2048) {
-        const dim3 gb((unsigned)(M / (BRX * BRY * BRZ)), c->cout / (16 * bnt));
-        switch (bnt) {
-            case 1: hipLaunchKernelGGL(k_conv_fwd_brick<1>, gb, dim3(256), halo_bytes, st, *c, K); break;
-            case 2: hipLaunchKernelGGL(k_conv_fwd_brick<2>, gb, dim3(256), halo_bytes, st, *c, K); break;
-            case 3: hipLaunchKernelGGL(k_conv_fwd_brick<3>, gb, dim3(256), halo_bytes, st, *c, K); break;
-            default: hipLaunchKernelGGL(k_conv_fwd_brick<4>, gb, dim3(256), halo_bytes, st, *c, K); break;
+    const bool bok = bnt && c->ntaps == 27 && !(c->flags & PVNET_GATHER) && c->s.d % BRZ == 0 && c->s.w % BRX == 0 &&
+                     M >= (int64_t)256 * 2048;
+    const bool by8 = bok && c->s.h % 8 == 0 && halo8 <= 64 * 1024;
+    const bool by4 = bok && !by8 && c->s.h % 4 == 0 && halo4 <= 64 * 1024;
+    if (by8 || by4) {
+        const unsigned nbr = (unsigned)(M / (BRX * (by8 ? 8 : 4) * BRZ));
+        const dim3 gb(nbr, c->cout / (16 * bnt));
+        const size_t lds = by8 ? halo8 : halo4;
+        if (by8) {
+            switch (bnt) {
+                case 1: hipLaunchKernelGGL((k_conv_fwd_brick<1, 8>), gb, dim3(512), lds, st, *c, K); break;
+                case 2: hipLaunchKernelGGL((k_conv_fwd_brick<2, 8>), gb, dim3(512), lds, st, *c, K); break;
+                case 3: hipLaunchKernelGGL((k_conv_fwd_brick<3, 8>), gb, dim3(512), lds, st, *c, K); break;
+                default: hipLaunchKernelGGL((k_conv_fwd_brick<4, 8>), gb, dim3(512), lds, st, *c, K); break;
+            }
+        } else {
+            switch (bnt) {
+                case 1: hipLaunchKernelGGL((k_conv_fwd_brick<1, 4>), gb, dim3(256), lds, st, *c, K); break;
+                case 2: hipLaunchKernelGGL((k_conv_fwd_brick<2, 4>), gb, dim3(256), lds, st, *c, K); break;
+                case 3: hipLaunchKernelGGL((k_conv_fwd_brick<3, 4>), gb, dim3(256), lds, st, *c, K); break;
+                default: hipLaunchKernelGGL((k_conv_fwd_brick<4, 4>), gb, dim3(256), lds, st, *c, K); break;
+            }
         }
     } else if (c->cout <= 16 && M >= (int64_t)256 * 2048) {
         hipLaunchKernelGGL((k_conv_fwd<16, 64>), dim3(blocks_for(M, 256), 1), dim3(256), 0, st, *c, (int)M, K);
