@@ -691,6 +691,25 @@ __device__ __forceinline__ int64_t coarse_index(const pvnet_shape& s, const Coar
     return ((o.b * (s.d / 2) + o.z) * (s.h / 2) + o.y) * (int64_t)(s.w / 2) + o.x;
 }
 
+// fine voxel f and 8-channel chunk of thread i (consecutive threads: the chunks of one voxel, then
+// the next voxel of the row), and the index of its parent coarse voxel; 32-bit arithmetic as above
+__device__ __forceinline__ bool fine_of(const pvnet_shape& s, int c8n, int64_t i, int64_t& f, int64_t& cidx, int& ch) {
+    const uint32_t w = s.w, h = s.h, d = s.d;
+    const int64_t n = (int64_t)s.nb * d * h * w * c8n;
+    if (i >= n) return false;
+    uint32_t v = (uint32_t)i;
+    ch = (int)(v % (uint32_t)c8n) * 8;
+    v /= (uint32_t)c8n;
+    f = v;
+    const uint32_t x = v % w;
+    v /= w;
+    const uint32_t y = v % h;
+    v /= h;
+    const uint32_t z = v % d, b = v / d;
+    cidx = (((int64_t)b * (d / 2) + z / 2) * (h / 2) + y / 2) * (int64_t)(w / 2) + x / 2;
+    return true;
+}
+
 __global__ void k_pool2_fwd(pvnet_shape s, int c, const uint16_t* x, int ldx, uint16_t* y, int ldy) {
     Coarse o;
     int ch;
@@ -708,48 +727,41 @@ __global__ void k_pool2_fwd(pvnet_shape s, int c, const uint16_t* x, int ldx, ui
     *(uint4*)(y + coarse_index(s, o) * ldy + ch) = out;
 }
 
-// relu_of (optional): the children's results are multiplied by (relu_of > 0), the backward of a
-// ReLU whose output is relu_of, in the same pass
+// one thread per fine voxel chunk: dx = dx (accumulate) + dy(parent) / 8.  relu_of (optional): the
+// result is multiplied by (relu_of > 0), the backward of a ReLU whose output is relu_of, in the same
+// pass
 __global__ void k_pool2_bwd(pvnet_shape s, int c, const uint16_t* dy, int lddy, uint16_t* dx, int lddx, int accum,
                             const uint16_t* relu_of, int ldr) {
-    Coarse o;
+    int64_t f, ci;
     int ch;
-    if (!coarse_of(s, c / 8, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, o, ch)) return;
-    const uint4 v = *(const uint4*)(dy + coarse_index(s, o) * lddy + ch);
+    if (!fine_of(s, c / 8, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, f, ci, ch)) return;
+    const uint4 v = *(const uint4*)(dy + ci * lddy + ch);
     const uint16_t* p = (const uint16_t*)&v;
-    float g[8];
-    for (int j = 0; j < 8; j++) g[j] = bf2f(p[j]) * 0.125f;
-    for (int k = 0; k < 8; k++) {
-        const int64_t f = fine_index(s, o.b, 2 * o.z + (k >> 2), 2 * o.y + ((k >> 1) & 1), 2 * o.x + (k & 1));
-        uint4* dst = (uint4*)(dx + f * lddx + ch);
-        uint4 out;
-        uint16_t* po = (uint16_t*)&out;
-        if (accum) {
-            const uint4 old = *dst;
-            const uint16_t* pold = (const uint16_t*)&old;
-            for (int j = 0; j < 8; j++) po[j] = f2bf(bf2f(pold[j]) + g[j]);
-        } else {
-            for (int j = 0; j < 8; j++) po[j] = f2bf(g[j]);
-        }
-        if (relu_of) {
-            const uint4 av = *(const uint4*)(relu_of + f * ldr + ch);
-            const uint16_t* pa = (const uint16_t*)&av;
-            for (int j = 0; j < 8; j++)
-                if (!bf_pos(pa[j])) po[j] = 0;
-        }
-        *dst = out;
+    uint4* dst = (uint4*)(dx + f * lddx + ch);
+    uint4 out;
+    uint16_t* po = (uint16_t*)&out;
+    if (accum) {
+        const uint4 old = *dst;
+        const uint16_t* pold = (const uint16_t*)&old;
+        for (int j = 0; j < 8; j++) po[j] = f2bf(bf2f(pold[j]) + bf2f(p[j]) * 0.125f);
+    } else {
+        for (int j = 0; j < 8; j++) po[j] = f2bf(bf2f(p[j]) * 0.125f);
     }
+    if (relu_of) {
+        const uint4 av = *(const uint4*)(relu_of + f * ldr + ch);
+        const uint16_t* pa = (const uint16_t*)&av;
+        for (int j = 0; j < 8; j++)
+            if (!bf_pos(pa[j])) po[j] = 0;
+    }
+    *dst = out;
 }
 
+// one thread per fine voxel chunk, copying its parent's
 __global__ void k_up2_fwd(pvnet_shape s, int c, const uint16_t* x, int ldx, uint16_t* y, int ldy) {
-    Coarse o;
+    int64_t f, ci;
     int ch;
-    if (!coarse_of(s, c / 8, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, o, ch)) return;
-    const uint4 v = *(const uint4*)(x + coarse_index(s, o) * ldx + ch);
-    for (int k = 0; k < 8; k++) {
-        const int64_t f = fine_index(s, o.b, 2 * o.z + (k >> 2), 2 * o.y + ((k >> 1) & 1), 2 * o.x + (k & 1));
-        *(uint4*)(y + f * ldy + ch) = v;
-    }
+    if (!fine_of(s, c / 8, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, f, ci, ch)) return;
+    *(uint4*)(y + f * ldy + ch) = *(const uint4*)(x + ci * ldx + ch);
 }
 
 __global__ void k_up2_bwd(pvnet_shape s, int c, const uint16_t* dy, int lddy, uint16_t* dx, int lddx) {
@@ -1034,7 +1046,7 @@ int pool_args(const pvnet_shape* s, int c, const void* a, int lda, const void* b
     if (!shape_ok(s) || s->d % 2 || s->h % 2 || s->w % 2) return fail(1, std::string(who) + ": sizes must be even");
     if (c <= 0 || c % 8 || lda < c || ldb < c || lda % 8 || ldb % 8) return fail(1, std::string(who) + ": bad channels");
     if (!a || !b || !al16(a) || !al16(b)) return fail(1, std::string(who) + ": null or misaligned pointer");
-    if (voxels(*s) / 8 * (c / 8) >= ((int64_t)1 << 31)) return fail(1, std::string(who) + ": too many voxels");
+    if (voxels(*s) * (c / 8) >= ((int64_t)1 << 31)) return fail(1, std::string(who) + ": too many voxels");
     return 0;
 }
 
@@ -1185,7 +1197,7 @@ int pvnet_pool2_fwd(const pvnet_shape* s, int32_t c, const uint16_t* x, int32_t 
 int pvnet_pool2_bwd(const pvnet_shape* s, int32_t c, const uint16_t* dy, int32_t lddy, uint16_t* dx, int32_t lddx,
                     int32_t accumulate, void* stream) {
     if (int rc = pool_args(s, c, dy, lddy, dx, lddx, "pvnet_pool2_bwd")) return rc;
-    const int64_t n = voxels(*s) / 8 * (c / 8);
+    const int64_t n = voxels(*s) * (c / 8);  // fine voxel chunks
     hipLaunchKernelGGL(k_pool2_bwd, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream, *s, c, dy, lddy, dx,
                        lddx, accumulate ? 1 : 0, (const uint16_t*)nullptr, 0);
     NET_LAUNCH();
@@ -1196,7 +1208,7 @@ int pvnet_pool2_bwd_relu(const pvnet_shape* s, int32_t c, const uint16_t* dy, in
                          int32_t lddx, const uint16_t* relu_of, int32_t ldr, void* stream) {
     if (int rc = pool_args(s, c, dy, lddy, dx, lddx, "pvnet_pool2_bwd_relu")) return rc;
     if (!relu_of || !al16(relu_of) || ldr < c || ldr % 8) return fail(1, "pvnet_pool2_bwd_relu: bad relu_of");
-    const int64_t n = voxels(*s) / 8 * (c / 8);
+    const int64_t n = voxels(*s) * (c / 8);  // fine voxel chunks
     hipLaunchKernelGGL(k_pool2_bwd, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream, *s, c, dy, lddy, dx,
                        lddx, 1, relu_of, ldr);
     NET_LAUNCH();
@@ -1206,7 +1218,7 @@ int pvnet_pool2_bwd_relu(const pvnet_shape* s, int32_t c, const uint16_t* dy, in
 int pvnet_up2_fwd(const pvnet_shape* s, int32_t c, const uint16_t* x, int32_t ldx, uint16_t* y, int32_t ldy,
                   void* stream) {
     if (int rc = pool_args(s, c, x, ldx, y, ldy, "pvnet_up2_fwd")) return rc;
-    const int64_t n = voxels(*s) / 8 * (c / 8);
+    const int64_t n = voxels(*s) * (c / 8);  // fine voxel chunks
     hipLaunchKernelGGL(k_up2_fwd, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream, *s, c, x, ldx, y, ldy);
     NET_LAUNCH();
     return 0;
