@@ -247,7 +247,9 @@ def main():
         "config": {
             "workload": workload,
             "lattice": q, "storage": args.storage, "box": list(box), "fluid_nodes": nfl,
-            "fluid_fraction": round(nfl / nbox, 4), "tau": tau,
+            "fluid_fraction": round(nfl / nbox, 4),
+            # the box's nodes per second: what a dense sweep of the whole box would have to reach
+            "box_sweep_mlups_equivalent": round(nbox * K / (ms * 1e-3) / 1e6, 1), "tau": tau,
             "inlet": "0.03+0.02*sin(2*pi*t/4000) x parabolic(mean 1)", "wss_every": args.wss_every,
             "wss_accumulators": ("every iteration" if args.wss_every == 1 else
                                  f"sampled every {args.wss_every} iterations (OSI/TAWSS sums over the samples)"),
