@@ -59,8 +59,9 @@ int pvnet_conv_fwd(const pvnet_conv* c, void* stream);
  * db[n]    = sum_m dy[m][n]                               (if db is not NULL)
  * ws: FP32 scratch of at least pvnet_wgrad_ws_floats(...) floats for the split-K partials; the
  * sums are reduced in a fixed order, so results are deterministic.  16- and 32-output 3x3x3 layers
- * on grids of whole 8 x 8 x 4 bricks reduce brick by brick with the input staged in LDS; flags = PVNET_GATHER keeps them on the per-tap gathering kernel (the same sums in
- * another order, for comparison).  flags = 0 otherwise. */
+ * on grids of whole 8 x 8 x 4 bricks reduce brick by brick with the input staged in LDS;
+ * flags = PVNET_GATHER keeps them on the per-tap gathering kernel (the same sums in another
+ * order, for comparison).  flags = 0 otherwise. */
 typedef struct {
     pvnet_shape s;
     int32_t ntaps, cin, cout;
