@@ -37,8 +37,9 @@ enum { PVNET_RELU = 1, PVNET_F32OUT = 2, PVNET_GATHER = 4 };
  * backward of a ReLU whose output is relu_of).  flags: PVNET_RELU applies max(., 0);
  * PVNET_F32OUT writes float instead of bf16.  y may be a channel slice of a wider tensor.
  * 3x3x3 convolutions with 16, 32, 48 or 64 outputs (or multiples of 48 or 64) on grids of whole
- * 16 x 4 x 4 bricks (x, y, z) stage each brick and its halo in LDS once; PVNET_GATHER keeps them on the per-tap gathering kernel (the
- * same result bit for bit, for comparison). */
+ * 16 x 4 x 4 bricks (x, y, z; 16 x 8 x 4 where the halo fits) stage each brick and its halo in
+ * LDS once; PVNET_GATHER keeps them on the per-tap gathering kernel (the same result bit for bit,
+ * for comparison). */
 typedef struct {
     pvnet_shape s;
     int32_t ntaps, cin, cout, kpad;
