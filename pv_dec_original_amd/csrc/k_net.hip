@@ -212,10 +212,13 @@ constexpr int hl_vox() { return HLX * (BY + 2) * HLZ; }
 // output voxel (chunk * HLV + tap shift), plus BR_TAB_BIAS; 0 where they lie beyond K
 constexpr int BR_TAB_BIAS = 1 << 20;
 
-template <int NT, int BY>
+// WL: the block's weight rows are staged in LDS after the step table (row pitch kpad + 8 bf16, an
+// odd number of 16-byte units, so the 16 rows one fragment reads fall on distinct banks), instead
+// of being read from global memory one step ahead.
+template <int NT, int BY, bool WL>
 __global__ __launch_bounds__(64 * BY) void k_conv_fwd_brick(pvnet_conv a, int K) {
     constexpr int HLY = BY + 2, HLV = hl_vox<BY>(), NTH = 64 * BY;
-    extern __shared__ uint4 halo[];  // [cin / 8 chunks][HLV voxels], then the step table
+    extern __shared__ uint4 halo[];  // [cin / 8 chunks][HLV voxels], the step table, WL: weights
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int zz = wv % BRZ, yq = wv / BRZ;  // this wave: plane zz, rows 4 yq .. 4 yq + 3
     const int g = lane >> 4, c16 = lane & 15;
@@ -244,6 +247,14 @@ __global__ __launch_bounds__(64 * BY) void k_conv_fwd_brick(pvnet_conv a, int K)
         }
         tab[i] = v;
     }
+    uint4* wl = reinterpret_cast<uint4*>(tab + 4 * nsteps);
+    const int WP = 4 * nsteps + 1;  // 16-byte units per weight row in LDS
+    if constexpr (WL) {
+        for (int i = t; i < 16 * NT * 4 * nsteps; i += NTH) {
+            const int row = i / (4 * nsteps), col = i - row * (4 * nsteps);
+            wl[row * WP + col] = *(const uint4*)(a.w + (int64_t)(n0 + row) * a.kpad + col * 8);
+        }
+    }
     for (int i = t; i < HLV * cch; i += NTH) {
         const int ch = i / HLV, v = i - ch * HLV;
         const int x = x0 - 1 + v % HLX, y = y0 - 1 + (v / HLX) % HLY, z = z0 - 1 + v / (HLX * HLY);
@@ -268,15 +279,22 @@ __global__ __launch_bounds__(64 * BY) void k_conv_fwd_brick(pvnet_conv a, int K)
     const uint16_t* wrow[NT];
 #pragma unroll
     for (int j = 0; j < NT; j++) wrow[j] = a.w + (int64_t)(n0 + 16 * j + c16) * a.kpad + g * 8;
+    // weight fragment of column tile j at step s
+    auto wfrag = [&](int j, int s) -> bf16x8 {
+        if constexpr (WL)
+            return *reinterpret_cast<const bf16x8*>(&wl[(16 * j + c16) * WP + 4 * s + g]);
+        else
+            return *(const bf16x8*)(wrow[j] + s * BK);
+    };
     int tv = tab[g];
     bf16x8 bfr[NT];
 #pragma unroll
-    for (int j = 0; j < NT; j++) bfr[j] = *(const bf16x8*)(wrow[j]);
+    for (int j = 0; j < NT; j++) bfr[j] = wfrag(j, 0);
     for (int s = 0; s + 1 < nsteps; s++) {
         const int tvn = tab[4 * (s + 1) + g];
         bf16x8 bn[NT];
 #pragma unroll
-        for (int j = 0; j < NT; j++) bn[j] = *(const bf16x8*)(wrow[j] + (s + 1) * BK);
+        for (int j = 0; j < NT; j++) bn[j] = wfrag(j, s + 1);
         const int base = vox + tv;
         uint4 r[4];
 #pragma unroll
@@ -1078,27 +1096,51 @@ int pvnet_conv_fwd(const pvnet_conv* c, void* stream) {
     const int bnt = c->cout % 16 ? 0 : c->cout <= 64 ? c->cout / 16 : c->cout % 48 == 0 ? 3 : c->cout % 64 == 0 ? 4 : 0;
     const bool bok = bnt && c->ntaps == 27 && !(c->flags & PVNET_GATHER) && c->s.d % BRZ == 0 && c->s.w % BRX == 0 &&
                      M >= (int64_t)256 * 2048;
-    const bool by8 = bok && c->s.h % 8 == 0 && halo8 <= 64 * 1024;
-    const bool by4 = bok && !by8 && c->s.h % 4 == 0 && halo4 <= 64 * 1024;
-    if (by8 || by4) {
+    // the first of (16 x 8 x 4, weights in LDS), (16 x 8 x 4), (16 x 4 x 4, weights in LDS),
+    // (16 x 4 x 4) whose LDS fits 64 KB: the taller brick first (more waves per CU for the same
+    // LDS), then the weights in LDS (no global load in the k-loop)
+    const size_t wbytes = (size_t)16 * bnt * (2 * (size_t)c->kpad + 16);
+    const size_t LIM = 64 * 1024;
+    bool by8 = false, wl = false;
+    size_t lds = 0;
+    if (bok && c->s.h % 8 == 0 && halo8 + wbytes <= LIM) {
+        by8 = wl = true;
+        lds = halo8 + wbytes;
+    } else if (bok && c->s.h % 8 == 0 && halo8 <= LIM) {
+        by8 = true;
+        lds = halo8;
+    } else if (bok && c->s.h % 4 == 0 && halo4 + wbytes <= LIM) {
+        wl = true;
+        lds = halo4 + wbytes;
+    } else if (bok && c->s.h % 4 == 0 && halo4 <= LIM) {
+        lds = halo4;
+    }
+    if (lds) {
         const unsigned nbr = (unsigned)(M / (BRX * (by8 ? 8 : 4) * BRZ));
         const dim3 gb(nbr, c->cout / (16 * bnt));
-        const size_t lds = by8 ? halo8 : halo4;
+#define PVNET_BRICK(NTV, BYV)                                                                         \
+    do {                                                                                              \
+        if (wl)                                                                                       \
+            hipLaunchKernelGGL((k_conv_fwd_brick<NTV, BYV, true>), gb, dim3(64 * BYV), lds, st, *c, K);  \
+        else                                                                                          \
+            hipLaunchKernelGGL((k_conv_fwd_brick<NTV, BYV, false>), gb, dim3(64 * BYV), lds, st, *c, K); \
+    } while (0)
         if (by8) {
             switch (bnt) {
-                case 1: hipLaunchKernelGGL((k_conv_fwd_brick<1, 8>), gb, dim3(512), lds, st, *c, K); break;
-                case 2: hipLaunchKernelGGL((k_conv_fwd_brick<2, 8>), gb, dim3(512), lds, st, *c, K); break;
-                case 3: hipLaunchKernelGGL((k_conv_fwd_brick<3, 8>), gb, dim3(512), lds, st, *c, K); break;
-                default: hipLaunchKernelGGL((k_conv_fwd_brick<4, 8>), gb, dim3(512), lds, st, *c, K); break;
+                case 1: PVNET_BRICK(1, 8); break;
+                case 2: PVNET_BRICK(2, 8); break;
+                case 3: PVNET_BRICK(3, 8); break;
+                default: PVNET_BRICK(4, 8); break;
             }
         } else {
             switch (bnt) {
-                case 1: hipLaunchKernelGGL((k_conv_fwd_brick<1, 4>), gb, dim3(256), lds, st, *c, K); break;
-                case 2: hipLaunchKernelGGL((k_conv_fwd_brick<2, 4>), gb, dim3(256), lds, st, *c, K); break;
-                case 3: hipLaunchKernelGGL((k_conv_fwd_brick<3, 4>), gb, dim3(256), lds, st, *c, K); break;
-                default: hipLaunchKernelGGL((k_conv_fwd_brick<4, 4>), gb, dim3(256), lds, st, *c, K); break;
+                case 1: PVNET_BRICK(1, 4); break;
+                case 2: PVNET_BRICK(2, 4); break;
+                case 3: PVNET_BRICK(3, 4); break;
+                default: PVNET_BRICK(4, 4); break;
             }
         }
+#undef PVNET_BRICK
     } else if (c->cout <= 16 && M >= (int64_t)256 * 2048) {
         hipLaunchKernelGGL((k_conv_fwd<16, 64>), dim3(blocks_for(M, 256), 1), dim3(256), 0, st, *c, (int)M, K);
     } else if (c->cout <= 16) {
