@@ -624,7 +624,9 @@ extern "C" int pvlbm_create(const uint8_t* fluid, const pvlbm_grid* grid, double
             }
             aux[n] = (int32_t)bnode.size();
             bnode.push_back(n);
-            bmask.push_back(m);
+            // bits 0..q-1: the missing links; bits 28..31: the closure (the step reads both at once)
+            const uint8_t closure = matched < 0 ? (uint8_t)BC_WALL : (uint8_t)(faces[matched].closure + 1);
+            bmask.push_back(m | ((uint32_t)closure << 28));
             if (matched < 0) {
                 kind[n] = K_WALL;
                 bclose.push_back(BC_WALL);

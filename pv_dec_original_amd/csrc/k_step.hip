@@ -208,15 +208,18 @@ __device__ __forceinline__ bool node_update(const StepParams& p, double (&red)[4
     const bool init = (p.flags & F_INIT) != 0;
     const bool bnd = (T.bnd & bit) != 0;
     const int32_t bslot = T.bslot + lanes_below(T.bnd);
-    // In WSS launches a boundary node's closure, inlet profile and outlet density and a next-to-wall
-    // node's previous macros, normal and accumulators are loaded here, before the pulls, so that
-    // the wave waits for memory once rather than again after them (3-4% on the D3Q27 FP64 and
-    // 16-bit WSS launches, +0.9% on D3Q19 FP64).  In the other launches the closure data is read
-    // after the pulls: loading it early there cost 0.5-0.8%.
-    const uint32_t nmiss = bnd ? p.bmask[bslot] : 0u;
-    const uint8_t bcl = (WSS && bnd) ? p.bclose[bslot] : (uint8_t)0;
+    // A boundary node's word holds its missing links (bits 0..Q-1) and its closure (bits 28..31),
+    // so a wall node needs no further load after the pulls.  In WSS launches the inlet profile,
+    // outlet density and a next-to-wall node's previous macros, normal and accumulators are loaded
+    // here too, before the pulls, so that the wave waits for memory once rather than again after
+    // them (3-4% on the D3Q27 FP64 and 16-bit WSS launches, +0.9% on D3Q19 FP64).  In the other
+    // launches profile and density are read after the pulls by the few inlet and outlet nodes.
+    static_assert(Q <= 28, "the closure shares the boundary word with the link bits");
+    const uint32_t bword = bnd ? p.bmask[bslot] : 0u;
+    const uint32_t nmiss = bword & ((1u << Q) - 1u);
+    const uint8_t bcl = (uint8_t)(bword >> 28);
     const double bprof = (WSS && bnd) ? p.bprof[bslot] : 0.0, brho = (WSS && bnd) ? p.brho[bslot] : 0.0;
-    auto closure = [&]() -> uint8_t { return WSS ? bcl : p.bclose[bslot]; };
+    auto closure = [&]() -> uint8_t { return bcl; };
     auto profile = [&]() -> double { return WSS ? bprof : p.bprof[bslot]; };
     auto density = [&]() -> double { return WSS ? brho : p.brho[bslot]; };
     const bool ntw = (T.ntw & bit) != 0;
