@@ -66,6 +66,15 @@ def build(verbose=False):
     os.replace(exe + ".tmp", exe)
 
 
+def build_test_shims(verbose=False):
+    """Compile the test shim that drives the C++ facade's loop in-process (test infrastructure)."""
+    src = os.path.join(ROOT, "tests", "facade", "facade_iterate.cpp")
+    out = os.path.join(ROOT, "tests", "facade", "libfacade_iterate.so")
+    _run(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-Wall", "-I", os.path.join(ROOT, "include"), src,
+          "-L", HERE, "-lpvlbm", "-Wl,-rpath,$ORIGIN/../../pv_dec_original_amd", "-o", out + ".tmp"], verbose)
+    os.replace(out + ".tmp", out)
+
+
 def build_oracle(verbose=False):
     """Compile the CPU oracle (test infrastructure) with its Makefile."""
     _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], verbose)
@@ -74,3 +83,4 @@ def build_oracle(verbose=False):
 if __name__ == "__main__":
     build(verbose=True)
     build_oracle(verbose=True)
+    build_test_shims(verbose=True)
