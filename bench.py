@@ -103,6 +103,9 @@ def main():
                     help="ybif: the Y-bifurcation; box: an all-fluid box (kernel ceiling); "
                          "aneurysm: the curved tube with a saccular bulge of the learned-start case")
     ap.add_argument("--wss-every", type=int, default=100)
+    ap.add_argument("--streaming", default="auto", choices=["auto", "in_place", "two_lattices"],
+                    help="in place on one lattice or the pull from one lattice into another (default: the "
+                         "library's choice by lattice size)")
     ap.add_argument("--ref-cadence-steps", type=int, default=2000,
                     help="also time this many steps with WSS every iteration, the reference's cadence "
                          "(0 = skip)")
@@ -145,6 +148,8 @@ def main():
     t0 = time.perf_counter()
     sd = SubDomain(case.geometry, case.faces, storage=args.storage, tau=tau, dt=8e-6)
     sim = Simulation(sd, tau, 1.0, 8e-6)
+    if args.streaming != "auto":
+        sim.set_streaming(N.STREAMING_IN_PLACE if args.streaming == "in_place" else N.STREAMING_TWO_LATTICES)
     t_setup = time.perf_counter() - t0
     counts = sd.counts()
     nfl = counts["fluid"]
@@ -254,6 +259,7 @@ def main():
             "wss_accumulators": ("every iteration" if args.wss_every == 1 else
                                  f"sampled every {args.wss_every} iterations (OSI/TAWSS sums over the samples)"),
             "lists": counts, "population_stores": "nontemporal" if sim.streaming_stores() else "plain",
+            "streaming": "in place, one lattice" if sim.in_place else "pull, two lattices",
             "wall_s_timed": round(wall, 3), "setup_s": round(t_setup, 2),
             "voxelise_s": round(t_geo, 2),
             "voxelise_kernel_ms": round(vox_ms.value, 3), "ray_triangle_tests": vox_tests.value,

@@ -250,14 +250,37 @@ int pvlbm_link_fractions(pvlbm_handle h, int ndirs, double* q_out);
 enum { PVLBM_WALL_REFERENCE = 0, PVLBM_WALL_INTERPOLATED = 1 };
 int pvlbm_set_wall_closure(pvlbm_handle h, int mode);
 
+/* Streaming scheme of the step (no counterpart in the reference, whose Streaming copies each
+ * population into the neighbour's array slot, Simulation.cpp:211-237; results are identical under
+ * every scheme).  In place, on one lattice (the AA pattern: alternate launches read and write each
+ * node's own slots, then its neighbours' slots), a step moves the same bytes as the pull from one
+ * lattice into another but touches half the memory, so a lattice near the size of the MI355X's
+ * 256 MB last-level cache is read from that cache; far beyond it the pull is faster.
+ * PVLBM_STREAMING_AUTO (default): in place while the lattice, Q * bytes per population * nodes,
+ * is at most PVLBM_STREAMING_AUTO_INPLACE_MAX_BYTES (256 MiB), the pull beyond.
+ * PVLBM_STREAMING_TWO_LATTICES / _IN_PLACE force one (in place halves the population memory, for
+ * domains too large for two lattices).  The interpolated wall closure (pvlbm_set_wall_closure)
+ * reads the node's previous populations from its own slots and always takes the pull; it cannot
+ * be combined with PVLBM_STREAMING_IN_PLACE (PVLBM_EINVAL).  A change of scheme moves the current
+ * state into the new layout at once (the second lattice is allocated when first needed);
+ * pvlbm_get_streaming reports the scheme in use (1 = in place). */
+enum { PVLBM_STREAMING_AUTO = 0, PVLBM_STREAMING_TWO_LATTICES = 1, PVLBM_STREAMING_IN_PLACE = 2 };
+#define PVLBM_STREAMING_AUTO_INPLACE_MAX_BYTES (256ll << 20)
+int pvlbm_set_streaming(pvlbm_handle h, int mode);
+int pvlbm_get_streaming(pvlbm_handle h, int* in_place);
+
 /* How the step writes the populations (no counterpart in the reference; results are identical
- * under every policy).  PVLBM_STORES_AUTO (default): nontemporal stores when one step's
- * algorithmic traffic, 2 * Q * bytes per population * nodes, exceeds PVLBM_STORES_AUTO_BYTES
- * (128 MiB, half the MI355X's 256 MB last-level cache), plain stores below it, whose written
- * lattice the next step then pulls from that cache.  PVLBM_STORES_REGULAR / _STREAMING force
- * one.  pvlbm_get_store_policy reports the stores in use (1 = nontemporal). */
+ * under every policy).  PVLBM_STORES_AUTO (default): plain stores while the lattice the next
+ * launch reads can stay in the MI355X's 256 MB last-level cache, so that launch reads it from
+ * there, and nontemporal stores beyond, which keep the written populations from evicting the
+ * lines the gathers are about to read: in place, nontemporal when the one lattice, Q * bytes per
+ * population * nodes, exceeds PVLBM_STORES_AUTO_INPLACE_BYTES (160 MiB); with two lattices, when
+ * one step's traffic, 2 * Q * bytes per population * nodes, exceeds PVLBM_STORES_AUTO_BYTES
+ * (128 MiB).  PVLBM_STORES_REGULAR / _STREAMING force one.  pvlbm_get_store_policy reports the
+ * stores in use (1 = nontemporal). */
 enum { PVLBM_STORES_AUTO = 0, PVLBM_STORES_REGULAR = 1, PVLBM_STORES_STREAMING = 2 };
 #define PVLBM_STORES_AUTO_BYTES (128ll << 20)
+#define PVLBM_STORES_AUTO_INPLACE_BYTES (160ll << 20)
 int pvlbm_set_store_policy(pvlbm_handle h, int policy);
 int pvlbm_get_store_policy(pvlbm_handle h, int* streaming);
 

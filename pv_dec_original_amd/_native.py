@@ -25,6 +25,7 @@ SOLID, FLUID, NTW, KINLET, KOUTLET, WALL = 0, 1, 2, 3, 4, 5
 LIST_FLUID, LIST_INLET, LIST_OUTLET1, LIST_OUTLET2, LIST_OUTLET3, LIST_WALL, LIST_NTW, LIST_TILES = range(8)
 WALL_REFERENCE, WALL_INTERPOLATED = 0, 1
 STORES_AUTO, STORES_REGULAR, STORES_STREAMING = 0, 1, 2
+STREAMING_AUTO, STREAMING_TWO_LATTICES, STREAMING_IN_PLACE = 0, 1, 2
 
 _d = C.POINTER(C.c_double)
 _u8 = C.POINTER(C.c_uint8)
@@ -104,6 +105,8 @@ PVLBM_API = [
     ("pvlbm_set_wall_closure", C.c_int, [C.c_void_p, C.c_int]),
     ("pvlbm_set_store_policy", C.c_int, [C.c_void_p, C.c_int]),
     ("pvlbm_get_store_policy", C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    ("pvlbm_set_streaming", C.c_int, [C.c_void_p, C.c_int]),
+    ("pvlbm_get_streaming", C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     ("pvlbm_step", C.c_int, [C.c_void_p, C.c_int, _d]),
     ("pvlbm_step_timed", C.c_int, [C.c_void_p, C.c_int, _d, _d]),
     ("pvlbm_iteration", C.c_int, [C.c_void_p, _i64]),

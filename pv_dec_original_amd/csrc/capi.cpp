@@ -138,6 +138,11 @@ struct pvlbm_sim {
     std::vector<uint8_t> bclose;
     int wall_mode = PVLBM_WALL_REFERENCE;
     int store_policy = PVLBM_STORES_AUTO;
+    int streaming = PVLBM_STREAMING_AUTO;
+    // kind of the launch that left the state (enum Scheme): SCH_PULL, state in A; SCH_EVEN /
+    // SCH_ODD, in place in A
+    int layout = SCH_PULL;
+    size_t pop_bytes = 0;
     int64_t nfluid = 0;
     float* d_bq = nullptr;
     // host-side lists (unpadded box indices) and slots
@@ -150,9 +155,18 @@ struct pvlbm_sim {
     size_t store_bytes() const { return storage == PVLBM_FP64 ? 8 : 2; }
     bool stream_stores() const {
         if (store_policy != PVLBM_STORES_AUTO) return store_policy == PVLBM_STORES_STREAMING;
+        if (in_place()) return (int64_t)(q * store_bytes()) * nfluid > PVLBM_STORES_AUTO_INPLACE_BYTES;
         return (int64_t)(2 * q * store_bytes()) * nfluid > PVLBM_STORES_AUTO_BYTES;
     }
     size_t real_bytes() const { return storage == PVLBM_FP64 ? 8 : 4; }
+    // in place on one lattice when asked, or by default while that lattice is within the cache
+    // budget; never under the interpolated closure, which reads the node's previous populations
+    // from its own slots
+    bool in_place() const {
+        if (wall_mode != PVLBM_WALL_REFERENCE || streaming == PVLBM_STREAMING_TWO_LATTICES) return false;
+        if (streaming == PVLBM_STREAMING_IN_PLACE) return true;
+        return (int64_t)(q * store_bytes()) * nfluid <= PVLBM_STREAMING_AUTO_INPLACE_MAX_BYTES;
+    }
     int blocks() const { return (ntiles + 3) / 4; }
     // padded row index of box row r = j + ny*k
     int64_t prow(int64_t r) const { return r % ny + nyp * (r / ny); }
@@ -186,7 +200,11 @@ struct pvlbm_sim {
             const int* cy = (q == 19) ? Lat<19>::cy : Lat<27>::cy;
             const int* cz = (q == 19) ? Lat<19>::cz : Lat<27>::cz;
             p.poff_plane = ns * S;
-            for (int d = 0; d < q; d++) p.poff_pull[d] = d * p.poff_plane - (cx[d] + cy[d] * p.sy + cz[d] * p.sz) * S;
+            for (int d = 0; d < q; d++) {
+                const int64_t o = (cx[d] + cy[d] * p.sy + cz[d] * p.sz) * S;
+                p.poff_pull[d] = d * p.poff_plane - o;
+                p.poff_push[d] = d * p.poff_plane + o;
+            }
         }
         p.tdesc = d_tdesc;
         p.ntiles = ntiles;
@@ -213,12 +231,47 @@ struct pvlbm_sim {
         return p;
     }
 
+    // The second lattice, allocated when a scheme first needs it (the pull, or a change of layout).
+    void ensure_second() {
+        if (d_pop[1]) return;
+        d_pop[1] = dalloc<char>(pop_bytes);
+        HIPCHK(hipMemsetAsync(d_pop[1], 0, pop_bytes, stream));
+        B = (A == d_pop[0]) ? d_pop[1] : d_pop[0];
+    }
+
+    // Brings the state into the layout of the scheme in use (values only move: fields unchanged).
+    void sync_layout() {
+        const bool want = in_place();
+        if (want == (layout != SCH_PULL)) return;
+        ensure_second();
+        StepParams p = params(0, 1.0);
+        const int to = want ? SCH_EVEN : SCH_PULL;
+        HIPCHK(launch_relayout(q, storage, p, layout, to, stream));
+        std::swap(A, B);
+        layout = to;
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+
     void launch(int flags, double scale, const double* init_rho = nullptr, const double* init_u = nullptr) {
         StepParams p = params(flags, scale);
         p.init_rho = init_rho;
         p.init_u = init_u;
-        HIPCHK(launch_step(q, storage, p, stream));
-        std::swap(A, B);
+        if (in_place()) {
+            // a start is an even launch without the gathers; then odd and even alternate
+            p.scheme = ((flags & F_INIT) || layout == SCH_ODD) ? SCH_EVEN : SCH_ODD;
+            if (!(flags & F_INIT) && layout == SCH_PULL) fail(PVLBM_ESTATE, "internal: state not in place");
+            p.B = A;
+            HIPCHK(launch_step(q, storage, p, stream));
+            layout = p.scheme;
+        } else {
+            ensure_second();
+            if (!(flags & F_INIT) && layout != SCH_PULL) fail(PVLBM_ESTATE, "internal: state in place");
+            p.B = B;
+            p.scheme = SCH_PULL;
+            HIPCHK(launch_step(q, storage, p, stream));
+            std::swap(A, B);
+            layout = SCH_PULL;
+        }
         if (flags & F_REDUCE) {
             HIPCHK(launch_reduce_final(d_partials, blocks(), d_red, stream));
             red_valid = true;
@@ -734,13 +787,13 @@ extern "C" int pvlbm_create(const uint8_t* fluid, const pvlbm_grid* grid, double
         S->d_partials = dalloc<double>((size_t)(4 * std::max(S->blocks(), 1)));
         S->d_red = dalloc<double>(4);
         S->d_gate = dalloc<unsigned long long>(1);
-        const size_t pop_bytes = (size_t)q * (size_t)S->ns * S->store_bytes();
-        for (int c = 0; c < 2; c++) {
-            S->d_pop[c] = dalloc<char>(pop_bytes);
-            HIPCHK(hipMemsetAsync(S->d_pop[c], 0, pop_bytes, st));
-        }
+        // one lattice for the in-place scheme; the second comes with the first launch of a scheme
+        // that needs it
+        S->pop_bytes = (size_t)q * (size_t)S->ns * S->store_bytes();
+        S->d_pop[0] = dalloc<char>(S->pop_bytes);
+        HIPCHK(hipMemsetAsync(S->d_pop[0], 0, S->pop_bytes, st));
         S->A = S->d_pop[0];
-        S->B = S->d_pop[1];
+        S->B = nullptr;
         HIPCHK(hipMemcpyAsync(S->d_aux, aux.data(), sizeof(int32_t) * nsp, hipMemcpyHostToDevice, st));
         if (!tdesc.empty())
             HIPCHK(hipMemcpyAsync(S->d_tdesc, tdesc.data(), sizeof(TileDesc) * tdesc.size(), hipMemcpyHostToDevice, st));
@@ -1123,6 +1176,8 @@ extern "C" int pvlbm_set_wall_closure(pvlbm_handle h, int mode) {
     if (!h) return PVLBM_EINVAL;
     return guarded(h, [&] {
         if (mode != PVLBM_WALL_REFERENCE && mode != PVLBM_WALL_INTERPOLATED) fail(PVLBM_EINVAL, "unknown wall closure");
+        if (mode == PVLBM_WALL_INTERPOLATED && h->streaming == PVLBM_STREAMING_IN_PLACE)
+            fail(PVLBM_EINVAL, "the interpolated wall closure needs two lattices (streaming is set to in place)");
         if (mode == PVLBM_WALL_INTERPOLATED && !h->d_bq) {
             // fractions of every link of every wall slot, per link length, nearest crossing of
             // the segment shifted like the voxelisation's test points
@@ -1152,6 +1207,27 @@ extern "C" int pvlbm_set_wall_closure(pvlbm_handle h, int mode) {
             HIPCHK(hipStreamSynchronize(h->stream));
         }
         h->wall_mode = mode;
+        h->sync_layout();
+    });
+}
+
+extern "C" int pvlbm_set_streaming(pvlbm_handle h, int mode) {
+    if (!h) return PVLBM_EINVAL;
+    return guarded(h, [&] {
+        if (mode != PVLBM_STREAMING_AUTO && mode != PVLBM_STREAMING_TWO_LATTICES && mode != PVLBM_STREAMING_IN_PLACE)
+            fail(PVLBM_EINVAL, "unknown streaming scheme %d", mode);
+        if (mode == PVLBM_STREAMING_IN_PLACE && h->wall_mode == PVLBM_WALL_INTERPOLATED)
+            fail(PVLBM_EINVAL, "the interpolated wall closure needs two lattices");
+        h->streaming = mode;
+        h->sync_layout();
+    });
+}
+
+extern "C" int pvlbm_get_streaming(pvlbm_handle h, int* in_place) {
+    if (!h) return PVLBM_EINVAL;
+    return guarded(h, [&] {
+        if (!in_place) fail(PVLBM_EINVAL, "in_place is NULL");
+        *in_place = h->in_place() ? 1 : 0;
     });
 }
 

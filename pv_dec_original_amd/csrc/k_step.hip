@@ -1,11 +1,15 @@
 // k_step.hip — the fused lattice Boltzmann step for gfx950.
 //
 // One launch = the streaming and boundary closures of reference iteration n-1 followed by the
-// collision of iteration n (Simulation::Iterate body, Simulation.cpp:577-586, regrouped): every
-// node pulls its populations from the post-collision lattice A and writes its own post-collision
-// populations to B.  A link with no upstream node keeps its previous post-closure value (the
-// reference does not stream into it, Simulation.cpp:218-224); that value is parked in the lattice
-// slot the link pulls from, which is not a node, so every lane does the same branch-free gather.
+// collision of iteration n (Simulation::Iterate body, Simulation.cpp:577-586, regrouped).  Two
+// schemes (enum Scheme): the pull from the post-collision lattice A into B, or the in-place AA
+// pattern on one lattice, whose even launches read and write each node's own slots (directions
+// reversed) and whose odd launches read from and write to the neighbours' slots; each node then
+// owns the slots it touches, so no launch needs a second lattice, and the working set per step
+// is half the pull's.  A link with no upstream node keeps its previous post-closure value (the
+// reference does not stream into it, Simulation.cpp:218-224); that value is parked in the slot
+// the next launch reads for that link, which in both schemes is a slot no other node touches,
+// so every lane does the same branch-free gather.
 // Boundary nodes then apply their closure (Zou-He or the wall rule); next-to-wall nodes evaluate
 // WSS against the equilibrium of their previous macros M (Simulation.cpp:880-898).
 //
@@ -71,6 +75,30 @@ __device__ __forceinline__ int32_t lanes_below(uint64_t m) {
     return (int32_t)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
+// Byte offset from a lattice base of direction D's slot for the node at byte index 0, by kind of
+// launch: the slot it reads, the slot its relaxed population goes to, and the slot a link of D
+// without an upstream node is parked in (the slot the next launch reads for D).
+template <int Q, int SCH, int D>
+__device__ __forceinline__ int64_t load_off(const StepParams& p) {
+    if constexpr (SCH == SCH_PULL) return p.poff_pull[D];       // plane D at x - c_D
+    else if constexpr (SCH == SCH_EVEN) return D * p.poff_plane;  // plane D at x
+    else return p.poff_push[Lat<Q>::opp[D]];                     // plane opp D at x - c_D
+}
+
+template <int Q, int SCH, int D>
+__device__ __forceinline__ int64_t store_off(const StepParams& p) {
+    if constexpr (SCH == SCH_PULL) return D * p.poff_plane;                    // plane D at x
+    else if constexpr (SCH == SCH_EVEN) return Lat<Q>::opp[D] * p.poff_plane;  // plane opp D at x
+    else return p.poff_push[D];                                                // plane D at x + c_D
+}
+
+template <int Q, int SCH, int D>
+__device__ __forceinline__ int64_t park_off(const StepParams& p) {
+    if constexpr (SCH == SCH_PULL) return p.poff_pull[D];
+    else if constexpr (SCH == SCH_EVEN) return load_off<Q, SCH_ODD, D>(p);
+    else return load_off<Q, SCH_EVEN, D>(p);
+}
+
 // Addresses: `tb` is the byte index of the tile's first lane within a plane (wave-uniform, 64-bit)
 // and `lo` the lane's byte offset within the tile (lane * sizeof(S)), so that each access is a
 // 64-bit scalar base per direction plus the one 32-bit lane offset all directions share.
@@ -79,32 +107,37 @@ __device__ __forceinline__ S* at(P* base, int64_t sbytes, uint32_t lo) {
     return reinterpret_cast<S*>(reinterpret_cast<typename std::conditional<std::is_const<P>::value, const char, char>::type*>(base) + sbytes + lo);
 }
 
-// Interior lanes: all Q pulls issued back to back, direction D from slot x - c_D of plane D.
-template <int Q, class ST, int D = 0>
+// All Q gathers issued back to back, before any store of the lane (in the in-place scheme a lane
+// writes the slots it read).
+template <int Q, class ST, int SCH, int D = 0>
 __device__ __forceinline__ void load_all(const StepParams& p, int64_t tb, uint32_t lo, typename ST::S (&v)[Q]) {
     if constexpr (D < Q) {
-        v[D] = *at<const typename ST::S>(p.A, p.poff_pull[D] + tb, lo);
-        load_all<Q, ST, D + 1>(p, tb, lo, v);
+        v[D] = *at<const typename ST::S>(p.A, load_off<Q, SCH, D>(p) + tb, lo);
+        load_all<Q, ST, SCH, D + 1>(p, tb, lo, v);
     }
 }
 
-// Stale populations live in the lattice itself: a link of node x with no upstream node pulls from
-// slot x - c_D of plane D, a non-node slot (solid, or padding / guard outside the box) that only x
-// ever addresses in that plane, so x parks its post-closure value there for the next step.
-template <int Q, class ST, int D = 0>
+// Stale populations live in the lattice itself.  In the pull scheme a link of node x with no
+// upstream node reads slot x - c_D of plane D, a non-node slot (solid, or padding / guard outside
+// the box) that only x ever addresses in that plane; in the in-place scheme an odd launch reads it
+// from slot x - c_D of plane opp D (likewise only x's), an even launch from x's own slot of
+// plane D, which no neighbour writes since x - c_D is not a node.  x parks its post-closure value
+// there for the next launch.
+template <int Q, class ST, int SCH, int D = 0>
 __device__ __forceinline__ void save_stale(const StepParams& p, int64_t tb, uint32_t lo, uint32_t nmiss,
                                            const typename ST::R (&f)[Q]) {
     if constexpr (D < Q) {
         using S = typename ST::S;
-        if ((nmiss >> Lat<Q>::opp[D]) & 1u) *at<S>(p.B, p.poff_pull[D] + tb, lo) = ST::store(f[D], Lat<Q>::w[D]);
-        save_stale<Q, ST, D + 1>(p, tb, lo, nmiss, f);
+        if ((nmiss >> Lat<Q>::opp[D]) & 1u)
+            *at<S>(p.B, park_off<Q, SCH, D>(p) + tb, lo) = ST::store(f[D], Lat<Q>::w[D]);
+        save_stale<Q, ST, SCH, D + 1>(p, tb, lo, nmiss, f);
     }
 }
 
 // Interpolated bounce-back on the wall links of a wall node (Bouzidi, Firdaouss and Lallemand
 // 2001, linear): for each population D with no upstream node, I = opp D points at the wall and
 // q = bq[slot][I] is where the link meets it.  f*(x) is this node's post-collision population of
-// the previous launch, read from its own slot of A; f*_I(x - c_I) is the population it pulled in
+// the previous launch, read from its own slot of A (pull scheme only); f*_I(x - c_I) is the population it pulled in
 // direction I (valid when x - c_I, i.e. x + c_D, is a node).
 template <int Q, class ST, int D = 0>
 __device__ __forceinline__ void wall_interpolated(const StepParams& p, int64_t tb, uint32_t lo, uint32_t nmiss,
@@ -129,7 +162,7 @@ __device__ __forceinline__ void wall_interpolated(const StepParams& p, int64_t t
     }
 }
 
-template <int Q, class ST, bool NT, int D = 0>
+template <int Q, class ST, bool NT, int SCH, int D = 0>
 __device__ __forceinline__ void collide_store(const StepParams& p, int64_t tb, uint32_t lo, typename ST::R rho,
                                               const typename ST::R (&u)[3],
                                               const typename ST::R (&f)[Q], typename ST::R omega) {
@@ -138,10 +171,10 @@ __device__ __forceinline__ void collide_store(const StepParams& p, int64_t tb, u
         using S = typename ST::S;
         const R feq = feq_dir<Q, D>(rho, u);
         const R fo = f[D] + omega * (feq - f[D]);
-        S* dst = at<S>(p.B, D * p.poff_plane + tb, lo);
+        S* dst = at<S>(p.B, store_off<Q, SCH, D>(p) + tb, lo);
         if constexpr (NT) __builtin_nontemporal_store(ST::store(fo, Lat<Q>::w[D]), dst);
         else *dst = ST::store(fo, Lat<Q>::w[D]);
-        collide_store<Q, ST, NT, D + 1>(p, tb, lo, rho, u, f, omega);
+        collide_store<Q, ST, NT, SCH, D + 1>(p, tb, lo, rho, u, f, omega);
     }
 }
 
@@ -163,7 +196,7 @@ __device__ __forceinline__ void init_populations(const StepParams& p, int64_t n,
 // store loop together (each still in its own A order, so every sum rounds exactly as the
 // reference's loop nest does), and f_A dies once its relaxed value is stored, so no Q-long array
 // stays live next to f.
-template <int Q, class ST, bool NT, int D = 0>
+template <int Q, class ST, bool NT, int SCH, int D = 0>
 __device__ __forceinline__ void collide_store_wss(const StepParams& p, int64_t tb, uint32_t lo, typename ST::R rho,
                                                   const typename ST::R (&u)[3], const typename ST::R (&f)[Q],
                                                   typename ST::R omega, typename ST::R mrho,
@@ -174,7 +207,7 @@ __device__ __forceinline__ void collide_store_wss(const StepParams& p, int64_t t
         using S = typename ST::S;
         const R feq = feq_dir<Q, D>(rho, u);
         const R fo = f[D] + omega * (feq - f[D]);
-        S* dst = at<S>(p.B, D * p.poff_plane + tb, lo);
+        S* dst = at<S>(p.B, store_off<Q, SCH, D>(p) + tb, lo);
         if constexpr (NT) __builtin_nontemporal_store(ST::store(fo, Lat<Q>::w[D]), dst);
         else *dst = ST::store(fo, Lat<Q>::w[D]);
         // the rest direction has weight zero: its product is a signed zero, which leaves the
@@ -185,14 +218,17 @@ __device__ __forceinline__ void collide_store_wss(const StepParams& p, int64_t t
             s1 += fne * wss_weight<Q, R, 1, D>(n);
             s2 += fne * wss_weight<Q, R, 2, D>(n);
         }
-        collide_store_wss<Q, ST, NT, D + 1>(p, tb, lo, rho, u, f, omega, mrho, mu, n, s0, s1, s2);
+        collide_store_wss<Q, ST, NT, SCH, D + 1>(p, tb, lo, rho, u, f, omega, mrho, mu, n, s0, s1, s2);
     }
 }
 
 // WI: the wall nodes take the interpolated closure (a separate instance, so that the default
-// reference closure does not carry its registers).
-template <int Q, class ST, bool WSS, bool NT, bool WI>
+// reference closure does not carry its registers); it reads the node's previous populations from
+// its own slots, which exist only in the pull scheme.  SCH: the kind of launch (enum Scheme); an
+// F_INIT launch of the in-place scheme is an even one without the gathers.
+template <int Q, class ST, bool WSS, bool NT, bool WI, int SCH>
 __device__ __forceinline__ bool node_update(const StepParams& p, double (&red)[4]) {
+    static_assert(!WI || SCH == SCH_PULL, "the interpolated closure needs the pull scheme");
     using R = typename ST::R;
     using S = typename ST::S;
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -246,7 +282,7 @@ __device__ __forceinline__ bool node_update(const StepParams& p, double (&red)[4
     R f[Q];
     if (!init) {
         S v[Q];
-        load_all<Q, ST>(p, tb, lo, v);
+        load_all<Q, ST, SCH>(p, tb, lo, v);
 #pragma unroll
         for (int d = 0; d < Q; d++) f[d] = ST::load(v[d], Lat<Q>::w[d]);
         if constexpr (ST::kGate) {
@@ -287,9 +323,9 @@ __device__ __forceinline__ bool node_update(const StepParams& p, double (&red)[4
             R fw[Q];
 #pragma unroll
             for (int d = 0; d < Q; d++) fw[d] = R(Lat<Q>::w[d]);
-            save_stale<Q, ST>(p, tb, lo, nmiss, fw);
+            save_stale<Q, ST, SCH>(p, tb, lo, nmiss, fw);
         } else {
-            save_stale<Q, ST>(p, tb, lo, nmiss, f);
+            save_stale<Q, ST, SCH>(p, tb, lo, nmiss, f);
         }
     }
 
@@ -301,7 +337,7 @@ __device__ __forceinline__ bool node_update(const StepParams& p, double (&red)[4
     // Tiles without one (a wave-uniform test) take the plain store.
     if (WSS && !init && T.ntw != 0) {
         R s0 = R(0.0), s1 = R(0.0), s2 = R(0.0);
-        collide_store_wss<Q, ST, NT>(p, tb, lo, rho, u, f, R(p.omega), mr, mu, nv, s0, s1, s2);
+        collide_store_wss<Q, ST, NT, SCH>(p, tb, lo, rho, u, f, R(p.omega), mr, mu, nv, s0, s1, s2);
         if (ntw) {
             const R C = R(p.C);
             const R t0 = C * s0, t1 = C * s1, t2 = C * s2;
@@ -316,7 +352,7 @@ __device__ __forceinline__ bool node_update(const StepParams& p, double (&red)[4
             p.acc[3 * ms + ws] = pacc[3] + (double)W;
         }
     } else {
-        collide_store<Q, ST, NT>(p, tb, lo, rho, u, f, R(p.omega));
+        collide_store<Q, ST, NT, SCH>(p, tb, lo, rho, u, f, R(p.omega));
     }
     if (ntw) {
         M[ws] = rho;
@@ -343,10 +379,10 @@ __device__ __forceinline__ bool node_update(const StepParams& p, double (&red)[4
     return true;
 }
 
-template <int Q, class ST, bool WSS, bool NT, bool WI>
+template <int Q, class ST, bool WSS, bool NT, bool WI, int SCH>
 __global__ __launch_bounds__(256) void k_step(const StepParams p) {
     double red[4] = {0.0, 0.0, 0.0, 0.0};
-    node_update<Q, ST, WSS, NT, WI>(p, red);
+    node_update<Q, ST, WSS, NT, WI, SCH>(p, red);
     if (p.flags & F_REDUCE) {
         // FP64 block sums of (mass, inlet flux, outlet flux, non-finite count)
         __shared__ double s[4][4];
@@ -362,6 +398,34 @@ __global__ __launch_bounds__(256) void k_step(const StepParams p) {
             p.partials[4 * (int64_t)blockIdx.x + k] = ((s[0][k] + s[1][k]) + s[2][k]) + s[3][k];
         }
     }
+}
+
+// State moved between layouts, from p.A into p.B: node x's relaxed population of direction D from
+// where a launch of kind KF stored it to where a launch of kind KT stores it, and each parked link
+// likewise.  Only values move, so the fields are unchanged bit for bit.
+template <int Q, class S, int KF, int KT, int D = 0>
+__device__ __forceinline__ void relayout_dirs(const StepParams& p, int64_t tb, uint32_t lo, uint32_t nmiss) {
+    if constexpr (D < Q) {
+        *at<S>(p.B, store_off<Q, KT, D>(p) + tb, lo) = *at<const S>(p.A, store_off<Q, KF, D>(p) + tb, lo);
+        if ((nmiss >> Lat<Q>::opp[D]) & 1u)
+            *at<S>(p.B, park_off<Q, KT, D>(p) + tb, lo) = *at<const S>(p.A, park_off<Q, KF, D>(p) + tb, lo);
+        relayout_dirs<Q, S, KF, KT, D + 1>(p, tb, lo, nmiss);
+    }
+}
+
+template <int Q, class S, int KF, int KT>
+__global__ __launch_bounds__(256) void k_relayout(const StepParams p) {
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int t = blockIdx.x * 4 + wave;
+    if (t >= p.ntiles) return;
+    const TileDesc T = p.tdesc[t];
+    const int lane = threadIdx.x & 63;
+    const uint64_t bit = 1ull << lane;
+    if (!(T.node & bit)) return;
+    const int64_t tb = ((int64_t)T.tile * 64 + p.guard) * (int64_t)sizeof(S);
+    const uint32_t lo = (uint32_t)lane * (uint32_t)sizeof(S);
+    const uint32_t nmiss = (T.bnd & bit) ? (p.bmask[T.bslot + lanes_below(T.bnd)] & ((1u << Q) - 1u)) : 0u;
+    relayout_dirs<Q, S, KF, KT>(p, tb, lo, nmiss);
 }
 
 // Deterministic final sum of the block partials: fixed strided order, fixed tree.
@@ -380,40 +444,67 @@ __global__ __launch_bounds__(256) void k_reduce_final(const double* partials, in
     if (threadIdx.x < 4) out4[threadIdx.x] = s[threadIdx.x][0];
 }
 
-template <int Q, class ST, bool WI>
-void launch_qw(const StepParams& p, hipStream_t s) {
+template <int Q, class ST, bool WI, int SCH>
+void launch_qws(const StepParams& p, hipStream_t s) {
     const dim3 g(p.blocks), b(256);
-    // nontemporal stores keep a lattice larger than the caches from evicting the pulls; a
+    // nontemporal stores keep a lattice larger than the caches from evicting the gathers; a
     // per-step working set that fits the last-level cache is served from it with plain stores
     if (p.stream_stores) {
-        if (p.flags & F_WSS) hipLaunchKernelGGL((k_step<Q, ST, true, true, WI>), g, b, 0, s, p);
-        else hipLaunchKernelGGL((k_step<Q, ST, false, true, WI>), g, b, 0, s, p);
+        if (p.flags & F_WSS) hipLaunchKernelGGL((k_step<Q, ST, true, true, WI, SCH>), g, b, 0, s, p);
+        else hipLaunchKernelGGL((k_step<Q, ST, false, true, WI, SCH>), g, b, 0, s, p);
     } else {
-        if (p.flags & F_WSS) hipLaunchKernelGGL((k_step<Q, ST, true, false, WI>), g, b, 0, s, p);
-        else hipLaunchKernelGGL((k_step<Q, ST, false, false, WI>), g, b, 0, s, p);
+        if (p.flags & F_WSS) hipLaunchKernelGGL((k_step<Q, ST, true, false, WI, SCH>), g, b, 0, s, p);
+        else hipLaunchKernelGGL((k_step<Q, ST, false, false, WI, SCH>), g, b, 0, s, p);
     }
 }
 
 template <int Q, class ST>
-void launch_q(const StepParams& p, hipStream_t s) {
-    if (p.bq) launch_qw<Q, ST, true>(p, s);
-    else launch_qw<Q, ST, false>(p, s);
+hipError_t launch_q(const StepParams& p, hipStream_t s) {
+    if (p.bq) {
+        if (p.scheme != SCH_PULL) return hipErrorInvalidValue;
+        launch_qws<Q, ST, true, SCH_PULL>(p, s);
+    } else if (p.scheme == SCH_PULL) {
+        launch_qws<Q, ST, false, SCH_PULL>(p, s);
+    } else if (p.scheme == SCH_EVEN) {
+        launch_qws<Q, ST, false, SCH_EVEN>(p, s);
+    } else if (p.scheme == SCH_ODD) {
+        launch_qws<Q, ST, false, SCH_ODD>(p, s);
+    } else {
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <int Q, class S>
+hipError_t relayout_q(const StepParams& p, int from, int to, hipStream_t s) {
+    const dim3 g(p.blocks), b(256);
+    if (from == SCH_PULL && to == SCH_EVEN) hipLaunchKernelGGL((k_relayout<Q, S, SCH_PULL, SCH_EVEN>), g, b, 0, s, p);
+    else if (from == SCH_EVEN && to == SCH_PULL) hipLaunchKernelGGL((k_relayout<Q, S, SCH_EVEN, SCH_PULL>), g, b, 0, s, p);
+    else if (from == SCH_ODD && to == SCH_PULL) hipLaunchKernelGGL((k_relayout<Q, S, SCH_ODD, SCH_PULL>), g, b, 0, s, p);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
 }
 
 }  // namespace
 
 hipError_t launch_step(int q, int storage, const StepParams& p, hipStream_t s) {
     if (p.blocks <= 0) return hipSuccess;
+    if (p.scheme != SCH_PULL && p.A != p.B) return hipErrorInvalidValue;  // in place: one lattice
     if (q == 19) {
-        if (storage == 0) launch_q<19, StFP64>(p, s);
-        else if (storage == 1) launch_q<19, StFP16>(p, s);
-        else launch_q<19, StBF16>(p, s);
-    } else {
-        if (storage == 0) launch_q<27, StFP64>(p, s);
-        else if (storage == 1) launch_q<27, StFP16>(p, s);
-        else launch_q<27, StBF16>(p, s);
+        if (storage == 0) return launch_q<19, StFP64>(p, s);
+        if (storage == 1) return launch_q<19, StFP16>(p, s);
+        return launch_q<19, StBF16>(p, s);
     }
-    return hipGetLastError();
+    if (storage == 0) return launch_q<27, StFP64>(p, s);
+    if (storage == 1) return launch_q<27, StFP16>(p, s);
+    return launch_q<27, StBF16>(p, s);
+}
+
+hipError_t launch_relayout(int q, int storage, const StepParams& p, int from, int to, hipStream_t s) {
+    if (p.blocks <= 0) return hipSuccess;
+    if (p.A == p.B) return hipErrorInvalidValue;
+    if (q == 19) return storage == 0 ? relayout_q<19, double>(p, from, to, s) : relayout_q<19, uint16_t>(p, from, to, s);
+    return storage == 0 ? relayout_q<27, double>(p, from, to, s) : relayout_q<27, uint16_t>(p, from, to, s);
 }
 
 hipError_t launch_reduce_final(const double* partials, int nparts, double* out4, hipStream_t s) {

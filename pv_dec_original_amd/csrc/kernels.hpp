@@ -15,8 +15,16 @@ enum StepFlags : int {
     F_INIT = 8,    // no gather/closure: start from f = w or f = f_eq(init_rho, init_u)
 };
 
-// One launch of the fused step kernel: the stream of iteration n-1 (pull from A), its closures
-// and WSS, then the collision of iteration n into B.
+// One launch of the fused step kernel: the stream of iteration n-1, its closures and WSS, then the
+// collision of iteration n.  Kinds of launch (StepParams::scheme):
+//   SCH_PULL: two lattices; node x reads direction d from A[d][x - c_d] and writes B[d][x].
+//   SCH_EVEN / SCH_ODD: one lattice (B == A), alternating (the AA pattern): an even launch reads
+//   A[d][x] and writes A[opp d][x]; an odd launch reads A[opp d][x - c_d] and writes
+//   A[d][x + c_d].  In either kind the slots a node reads are the slots it writes, and no two
+//   nodes share one, so the update is in place.
+// A link with no upstream node parks its post-closure value where the next launch reads it.
+enum Scheme : int { SCH_PULL = 0, SCH_EVEN = 1, SCH_ODD = 2 };
+
 // One active 64-node x-segment of the padded box.  Lane l of the tile is node 64*tile + l; the
 // masks mark which lanes are lattice nodes, next-to-wall nodes and boundary (inlet/outlet/wall)
 // nodes.  Side-table slots are assigned in node order, so the slot of a lane is the first slot of
@@ -43,11 +51,14 @@ struct StepParams {
     // offset, with no 32-bit limit on the size of a plane.
     int64_t poff_plane;
     int64_t poff_pull[27];
+    int64_t poff_push[27];  // plane d, shifted by +c_d
     const TileDesc* tdesc;  // active tiles
     int32_t ntiles;
     int32_t blocks;         // launched blocks (4 tiles each)
     int32_t flags;
     int32_t stream_stores;  // 1: nontemporal population stores (per-step traffic above the cache)
+    int32_t scheme;         // enum Scheme
+    int32_t pad0;
     // boundary table (inlet, outlet and wall nodes)
     const uint32_t* bmask;  // bit d set: neighbour x + c_d is not a node
     const uint8_t* bclose;  // enum BClose
@@ -87,6 +98,9 @@ struct VoxParams {
 // storage codes match enum pvlbm_storage
 // one step over all active tiles
 hipError_t launch_step(int q, int storage, const StepParams& p, hipStream_t s);
+// Moves the state left by a launch of kind `from` to the layout a launch of kind `to` leaves, from
+// p.A into p.B (distinct arrays); from/to: SCH_PULL -> SCH_EVEN, or SCH_EVEN / SCH_ODD -> SCH_PULL.
+hipError_t launch_relayout(int q, int storage, const StepParams& p, int from, int to, hipStream_t s);
 hipError_t launch_reduce_final(const double* partials, int nparts, double* out4, hipStream_t s);
 hipError_t launch_voxelize(const VoxParams& p, hipStream_t s);
 // boundary flags: per padded node, 1 if fluid and a Q neighbour is missing (padded box nxp x nyp

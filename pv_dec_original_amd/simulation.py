@@ -197,14 +197,28 @@ class Simulation:
         self._chk(N.lib().pvlbm_set_wall_closure(self.handle, int(mode)))
 
     def set_store_policy(self, policy):
-        """N.STORES_AUTO (default: nontemporal population stores when one step's traffic exceeds
-        128 MiB), N.STORES_REGULAR or N.STORES_STREAMING; results are the same under each."""
+        """N.STORES_AUTO (default: nontemporal population stores when the lattice the next launch
+        reads exceeds the cache budget: in place Q*S*n > 160 MiB, two lattices 2*Q*S*n > 128 MiB),
+        N.STORES_REGULAR or N.STORES_STREAMING; results are the same under each."""
         self._chk(N.lib().pvlbm_set_store_policy(self.handle, int(policy)))
 
     def streaming_stores(self):
         """True when the step writes the populations with nontemporal stores."""
         v = C.c_int(0)
         self._chk(N.lib().pvlbm_get_store_policy(self.handle, C.byref(v)))
+        return bool(v.value)
+
+    def set_streaming(self, mode):
+        """N.STREAMING_AUTO (in place on one lattice while it is at most 256 MiB, the pull from one
+        lattice into another beyond, and always the pull under the interpolated wall closure),
+        N.STREAMING_TWO_LATTICES or N.STREAMING_IN_PLACE; same results under each."""
+        self._chk(N.lib().pvlbm_set_streaming(self.handle, int(mode)))
+
+    @property
+    def in_place(self):
+        """True when the step streams in place on one lattice."""
+        v = C.c_int(0)
+        self._chk(N.lib().pvlbm_get_streaming(self.handle, C.byref(v)))
         return bool(v.value)
 
     def set_wss_every(self, every):

@@ -72,6 +72,9 @@ def test_error_paths_do_not_throw():
     q = np.zeros(27)
     assert N.lib().pvlbm_link_fractions(None, 19, N.dptr(q)) == 1
     assert N.lib().pvlbm_set_wall_closure(None, N.WALL_INTERPOLATED) == 1
+    assert N.lib().pvlbm_set_streaming(None, N.STREAMING_TWO_LATTICES) == 1
+    v = C.c_int(7)
+    assert N.lib().pvlbm_get_streaming(None, C.byref(v)) == 1 and v.value == 7
     assert N.lib().pvlbm_init_fields_coarse(None, None, None, None, 0.3, 0.8, 1.25, 0.0, None, None, None) == 1
 
 

@@ -118,26 +118,34 @@ def test_bad_arguments_are_errors_and_leave_the_state():
 
 
 def test_store_policy_rule_and_identical_results():
-    """The automatic store policy writes populations with nontemporal stores when one step's
-    traffic 2 Q S n exceeds 128 MiB and with plain stores below; forcing either gives the same
-    fields bit for bit.  A 128 x 64 x 64 all-fluid box, D3Q19: 159 MB per step in FP64
-    (streaming), 40 MB in FP16 (plain)."""
+    """The automatic store policy writes populations with nontemporal stores when the lattice the
+    next launch reads outgrows the last-level cache: in place when Q S n exceeds 160 MiB, with two
+    lattices when one step's traffic 2 Q S n exceeds 128 MiB; plain stores below.  Forcing either
+    policy, in either scheme, gives the same fields bit for bit.  A 128 x 96 x 96 all-fluid box,
+    D3Q19: one lattice of 179 MB in FP64 (nontemporal in both schemes) and 45 MB in FP16 (plain
+    in both)."""
     from pv_dec_original_amd.simulation import Simulation, SubDomain
-    case = fluid_box_case(q=19, box=(128, 64, 64), tau=0.7)
-    n = 128 * 64 * 64
+    case = fluid_box_case(q=19, box=(128, 96, 96), tau=0.7)
+    n = 128 * 96 * 96
     fields = {}
-    for storage, auto in (("fp64", True), ("fp16", False)):
-        assert (2 * 19 * (8 if storage == "fp64" else 2) * n > (128 << 20)) == auto
-        for policy in (N.STORES_AUTO, N.STORES_REGULAR, N.STORES_STREAMING):
-            sd = SubDomain(case.geometry, case.faces, storage=storage, tau=case.tau, dt=G.DT, triangles=False)
-            sim = Simulation(sd, case.tau, 1.0, G.DT)
-            sim.set_inlet_profile(case.inlet_profile(sd, 0.05))
-            sim.set_store_policy(policy)
-            expect = {N.STORES_AUTO: auto, N.STORES_REGULAR: False, N.STORES_STREAMING: True}[policy]
-            assert sim.streaming_stores() == expect, (storage, policy)
-            sim.step(40)
-            fields[(storage, policy)] = sim.macros()
-        for policy in (N.STORES_REGULAR, N.STORES_STREAMING):
-            a, b = fields[(storage, N.STORES_AUTO)], fields[(storage, policy)]
-            assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), (storage, policy)
-    assert np.abs(fields[("fp64", N.STORES_AUTO)][1]).max() > 1e-3
+    for storage in ("fp64", "fp16"):
+        S = 8 if storage == "fp64" else 2
+        for scheme in (N.STREAMING_IN_PLACE, N.STREAMING_TWO_LATTICES):
+            auto = (19 * S * n > (160 << 20)) if scheme == N.STREAMING_IN_PLACE else (2 * 19 * S * n > (128 << 20))
+            assert auto == (storage == "fp64")
+            for policy in (N.STORES_AUTO, N.STORES_REGULAR, N.STORES_STREAMING):
+                sd = SubDomain(case.geometry, case.faces, storage=storage, tau=case.tau, dt=G.DT, triangles=False)
+                sim = Simulation(sd, case.tau, 1.0, G.DT)
+                sim.set_inlet_profile(case.inlet_profile(sd, 0.05))
+                sim.set_streaming(scheme)
+                sim.set_store_policy(policy)
+                expect = {N.STORES_AUTO: auto, N.STORES_REGULAR: False, N.STORES_STREAMING: True}[policy]
+                assert sim.streaming_stores() == expect, (storage, scheme, policy)
+                sim.step(40)
+                fields[(storage, scheme, policy)] = sim.macros()
+                del sim, sd
+        a = fields[(storage, N.STREAMING_IN_PLACE, N.STORES_AUTO)]
+        for key, b in fields.items():
+            if key[0] == storage:
+                assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), key
+    assert np.abs(fields[("fp64", N.STREAMING_IN_PLACE, N.STORES_AUTO)][1]).max() > 1e-3
