@@ -273,14 +273,16 @@ int pvlbm_get_streaming(pvlbm_handle h, int* in_place);
  * under every policy).  PVLBM_STORES_AUTO (default): plain stores while the lattice the next
  * launch reads can stay in the MI355X's 256 MB last-level cache, so that launch reads it from
  * there, and nontemporal stores beyond, which keep the written populations from evicting the
- * lines the gathers are about to read: in place, nontemporal when the one lattice, Q * bytes per
- * population * nodes, exceeds PVLBM_STORES_AUTO_INPLACE_BYTES (160 MiB); with two lattices, when
- * one step's traffic, 2 * Q * bytes per population * nodes, exceeds PVLBM_STORES_AUTO_BYTES
- * (128 MiB).  PVLBM_STORES_REGULAR / _STREAMING force one.  pvlbm_get_store_policy reports the
- * stores in use (1 = nontemporal). */
+ * lines the gathers are about to read: in place, nontemporal when the 128-byte lines of the one
+ * lattice that hold nodes (lines per plane x 128 B x Q) exceed PVLBM_STORES_AUTO_INPLACE_BYTES
+ * (240 MiB); with two lattices, when one step's traffic, 2 * Q * bytes per population * nodes,
+ * exceeds PVLBM_STORES_AUTO_BYTES (128 MiB).  In place with plain stores, every other launch
+ * sweeps the lattice in reverse, so that it starts on the lines the previous launch touched last.
+ * PVLBM_STORES_REGULAR / _STREAMING force one.  pvlbm_get_store_policy reports the stores in use
+ * (1 = nontemporal). */
 enum { PVLBM_STORES_AUTO = 0, PVLBM_STORES_REGULAR = 1, PVLBM_STORES_STREAMING = 2 };
 #define PVLBM_STORES_AUTO_BYTES (128ll << 20)
-#define PVLBM_STORES_AUTO_INPLACE_BYTES (160ll << 20)
+#define PVLBM_STORES_AUTO_INPLACE_BYTES (240ll << 20)
 int pvlbm_set_store_policy(pvlbm_handle h, int policy);
 int pvlbm_get_store_policy(pvlbm_handle h, int* streaming);
 

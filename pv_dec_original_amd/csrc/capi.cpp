@@ -144,6 +144,7 @@ struct pvlbm_sim {
     int layout = SCH_PULL;
     size_t pop_bytes = 0;
     int64_t nfluid = 0;
+    int64_t footprint = 0;  // bytes of the 128-byte lines of one lattice that hold nodes
     float* d_bq = nullptr;
     // host-side lists (unpadded box indices) and slots
     std::vector<int64_t> lists[8];
@@ -155,7 +156,7 @@ struct pvlbm_sim {
     size_t store_bytes() const { return storage == PVLBM_FP64 ? 8 : 2; }
     bool stream_stores() const {
         if (store_policy != PVLBM_STORES_AUTO) return store_policy == PVLBM_STORES_STREAMING;
-        if (in_place()) return (int64_t)(q * store_bytes()) * nfluid > PVLBM_STORES_AUTO_INPLACE_BYTES;
+        if (in_place()) return footprint > PVLBM_STORES_AUTO_INPLACE_BYTES;
         return (int64_t)(2 * q * store_bytes()) * nfluid > PVLBM_STORES_AUTO_BYTES;
     }
     size_t real_bytes() const { return storage == PVLBM_FP64 ? 8 : 4; }
@@ -261,6 +262,10 @@ struct pvlbm_sim {
             p.scheme = ((flags & F_INIT) || layout == SCH_ODD) ? SCH_EVEN : SCH_ODD;
             if (!(flags & F_INIT) && layout == SCH_PULL) fail(PVLBM_ESTATE, "internal: state not in place");
             p.B = A;
+            // with plain stores the lattice stays in the last-level cache between launches; odd
+            // launches sweep it backwards, so each launch starts on the lines the previous one
+            // touched last
+            p.reverse = (p.scheme == SCH_ODD && !p.stream_stores) ? 1 : 0;
             HIPCHK(launch_step(q, storage, p, stream));
             layout = p.scheme;
         } else {
@@ -760,6 +765,13 @@ extern "C" int pvlbm_create(const uint8_t* fluid, const pvlbm_grid* grid, double
                 }
             }
             if (bs != S->nb || wsl != (int32_t)wnode.size()) fail(PVLBM_EINVAL, "internal: slot order mismatch");
+            // cache footprint of one lattice: the 128-byte lines of a plane that hold a node, x Q
+            const int per = (int)(128 / S->store_bytes());
+            int64_t lines = 0;
+            for (const TileDesc& D : tdesc)
+                for (int l = 0; l < 64; l += per)
+                    if ((D.node >> l) & (per == 64 ? ~0ull : ((1ull << per) - 1))) lines++;
+            S->footprint = lines * 128 * q;
         }
         xyz.resize((size_t)(3 * S->nw));
         for (int64_t w = 0; w < S->nw; w++) {

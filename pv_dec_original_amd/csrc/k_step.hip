@@ -227,12 +227,12 @@ __device__ __forceinline__ void collide_store_wss(const StepParams& p, int64_t t
 // its own slots, which exist only in the pull scheme.  SCH: the kind of launch (enum Scheme); an
 // F_INIT launch of the in-place scheme is an even one without the gathers.
 template <int Q, class ST, bool WSS, bool NT, bool WI, int SCH>
-__device__ __forceinline__ bool node_update(const StepParams& p, double (&red)[4]) {
+__device__ __forceinline__ bool node_update(const StepParams& p, int blk, double (&red)[4]) {
     static_assert(!WI || SCH == SCH_PULL, "the interpolated closure needs the pull scheme");
     using R = typename ST::R;
     using S = typename ST::S;
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const int t = blockIdx.x * 4 + wave;
+    const int t = blk * 4 + wave;
     if (t >= p.ntiles) return false;
     const TileDesc T = p.tdesc[t];  // wave-uniform: scalar loads
     const int lane = threadIdx.x & 63;
@@ -379,10 +379,14 @@ __device__ __forceinline__ bool node_update(const StepParams& p, double (&red)[4
     return true;
 }
 
+// Block b takes tiles 4b .. 4b+3, or with p.reverse block b takes the tiles of block
+// blocks-1-b: workgroups start in index order, so the sweep runs from the other end of the
+// lattice, while each group of four tiles (and so each block partial sum) stays the same.
 template <int Q, class ST, bool WSS, bool NT, bool WI, int SCH>
 __global__ __launch_bounds__(256) void k_step(const StepParams p) {
     double red[4] = {0.0, 0.0, 0.0, 0.0};
-    node_update<Q, ST, WSS, NT, WI, SCH>(p, red);
+    const int blk = p.reverse ? p.blocks - 1 - (int)blockIdx.x : (int)blockIdx.x;
+    node_update<Q, ST, WSS, NT, WI, SCH>(p, blk, red);
     if (p.flags & F_REDUCE) {
         // FP64 block sums of (mass, inlet flux, outlet flux, non-finite count)
         __shared__ double s[4][4];
@@ -395,7 +399,7 @@ __global__ __launch_bounds__(256) void k_step(const StepParams p) {
         __syncthreads();
         if (threadIdx.x < 4) {
             const int k = threadIdx.x;
-            p.partials[4 * (int64_t)blockIdx.x + k] = ((s[0][k] + s[1][k]) + s[2][k]) + s[3][k];
+            p.partials[4 * (int64_t)blk + k] = ((s[0][k] + s[1][k]) + s[2][k]) + s[3][k];
         }
     }
 }

@@ -58,7 +58,7 @@ struct StepParams {
     int32_t flags;
     int32_t stream_stores;  // 1: nontemporal population stores (per-step traffic above the cache)
     int32_t scheme;         // enum Scheme
-    int32_t pad0;
+    int32_t reverse;        // 1: blocks take their tiles from the end (a sweep in the other direction)
     // boundary table (inlet, outlet and wall nodes)
     const uint32_t* bmask;  // bit d set: neighbour x + c_d is not a node
     const uint8_t* bclose;  // enum BClose

@@ -198,7 +198,8 @@ class Simulation:
 
     def set_store_policy(self, policy):
         """N.STORES_AUTO (default: nontemporal population stores when the lattice the next launch
-        reads exceeds the cache budget: in place Q*S*n > 160 MiB, two lattices 2*Q*S*n > 128 MiB),
+        reads exceeds the cache budget: in place its lines holding nodes > 240 MiB, two lattices
+        2*Q*S*n > 128 MiB),
         N.STORES_REGULAR or N.STORES_STREAMING; results are the same under each."""
         self._chk(N.lib().pvlbm_set_store_policy(self.handle, int(policy)))
 

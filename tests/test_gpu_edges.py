@@ -119,19 +119,19 @@ def test_bad_arguments_are_errors_and_leave_the_state():
 
 def test_store_policy_rule_and_identical_results():
     """The automatic store policy writes populations with nontemporal stores when the lattice the
-    next launch reads outgrows the last-level cache: in place when Q S n exceeds 160 MiB, with two
-    lattices when one step's traffic 2 Q S n exceeds 128 MiB; plain stores below.  Forcing either
-    policy, in either scheme, gives the same fields bit for bit.  A 128 x 96 x 96 all-fluid box,
-    D3Q19: one lattice of 179 MB in FP64 (nontemporal in both schemes) and 45 MB in FP16 (plain
-    in both)."""
+    next launch reads outgrows the last-level cache: in place when the lattice's lines that hold
+    nodes (Q S n for an all-fluid box of whole tiles) exceed 240 MiB, with two lattices when one
+    step's traffic 2 Q S n exceeds 128 MiB; plain stores below.  Forcing either policy, in either
+    scheme, gives the same fields bit for bit.  A 128 x 128 x 104 all-fluid box, D3Q19: one
+    lattice of 259 MB in FP64 (nontemporal in both schemes) and 65 MB in FP16 (plain in both)."""
     from pv_dec_original_amd.simulation import Simulation, SubDomain
-    case = fluid_box_case(q=19, box=(128, 96, 96), tau=0.7)
-    n = 128 * 96 * 96
+    case = fluid_box_case(q=19, box=(128, 128, 104), tau=0.7)
+    n = 128 * 128 * 104
     fields = {}
     for storage in ("fp64", "fp16"):
         S = 8 if storage == "fp64" else 2
         for scheme in (N.STREAMING_IN_PLACE, N.STREAMING_TWO_LATTICES):
-            auto = (19 * S * n > (160 << 20)) if scheme == N.STREAMING_IN_PLACE else (2 * 19 * S * n > (128 << 20))
+            auto = (19 * S * n > (240 << 20)) if scheme == N.STREAMING_IN_PLACE else (2 * 19 * S * n > (128 << 20))
             assert auto == (storage == "fp64")
             for policy in (N.STORES_AUTO, N.STORES_REGULAR, N.STORES_STREAMING):
                 sd = SubDomain(case.geometry, case.faces, storage=storage, tau=case.tau, dt=G.DT, triangles=False)
