@@ -20,8 +20,8 @@ enum StepFlags : int {
 //   SCH_PULL: two lattices; node x reads direction d from A[d][x - c_d] and writes B[d][x].
 //   SCH_EVEN / SCH_ODD: one lattice (B == A), alternating (the AA pattern): an even launch reads
 //   A[d][x] and writes A[opp d][x]; an odd launch reads A[opp d][x - c_d] and writes
-//   A[d][x + c_d].  In either kind the slots a node reads are the slots it writes, and no two
-//   nodes share one, so the update is in place.
+//   A[d][x + c_d].  In either kind a node's relaxed populations go back to the slots it read,
+//   and no two nodes address the same slot, so the update is in place.
 // A link with no upstream node parks its post-closure value where the next launch reads it.
 enum Scheme : int { SCH_PULL = 0, SCH_EVEN = 1, SCH_ODD = 2 };
 
