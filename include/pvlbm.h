@@ -156,7 +156,8 @@ int pvlbm_voxelize_stats(int64_t* tests, double* kernel_ms);
  * Node.cpp:30-37) and the first collision.  lattice is 19 or 27.  tau and dt play the roles of
  * Simulation(SubDomain&, tau, pressure, del_t) (Simulation.h:26); h enters the WSS constant as
  * the reference's spacing (Simulation.cpp:880).  Replaces SubDomain::SubDomain and the
- * set-up half of Simulation::Simulation. */
+ * set-up half of Simulation::Simulation.  The populations take one lattice of Q planes over the
+ * padded box (two when the streaming scheme takes the pull, pvlbm_set_streaming). */
 int pvlbm_create(const uint8_t* fluid, const pvlbm_grid* grid, double h, int lattice, int storage,
                  const pvlbm_face* faces, int nfaces, const double* tri, int64_t ntri, double tau,
                  double dt, pvlbm_handle* out);
@@ -340,8 +341,9 @@ int pvlbm_get_wss_raw(pvlbm_handle h, double* tau3, double* acc4);
  * outlet nodes, out[3] = number of nodes whose rho or u is not finite. */
 int pvlbm_reduce(pvlbm_handle h, double out[4]);
 
-/* Number of pulls that met the reference's "f_new != 0.0" stream gate (Simulation.cpp:222) on a
- * link whose stale value is not kept; nonzero means a run left the reference's semantics. */
+/* Number of gathered populations that met the reference's "f_new != 0.0" stream gate
+ * (Simulation.cpp:222) on a link whose stale value is not kept (FP64); nonzero means a run left the
+ * reference's semantics. */
 int pvlbm_gate_events(pvlbm_handle h, int64_t* n);
 
 /* Reference inlet waveform: piecewise polynomial of period `period`
