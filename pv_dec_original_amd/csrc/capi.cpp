@@ -248,9 +248,22 @@ struct pvlbm_sim {
         StepParams p = params(0, 1.0);
         const int to = want ? SCH_EVEN : SCH_PULL;
         HIPCHK(launch_relayout(q, storage, p, layout, to, stream));
+        HIPCHK(hipStreamSynchronize(stream));
         std::swap(A, B);
         layout = to;
-        HIPCHK(hipStreamSynchronize(stream));
+    }
+
+    // Sets a mode that the scheme depends on and moves the state into its layout; if that fails
+    // (no memory for the second lattice), the previous mode stays and the state is untouched.
+    void set_mode_and_sync(int& field, int value) {
+        const int old = field;
+        field = value;
+        try {
+            sync_layout();
+        } catch (...) {
+            field = old;
+            throw;
+        }
     }
 
     void launch(int flags, double scale, const double* init_rho = nullptr, const double* init_u = nullptr) {
@@ -1218,8 +1231,7 @@ extern "C" int pvlbm_set_wall_closure(pvlbm_handle h, int mode) {
             HIPCHK(hipMemcpyAsync(h->d_bq, bq.data(), sizeof(float) * bq.size(), hipMemcpyHostToDevice, h->stream));
             HIPCHK(hipStreamSynchronize(h->stream));
         }
-        h->wall_mode = mode;
-        h->sync_layout();
+        h->set_mode_and_sync(h->wall_mode, mode);
     });
 }
 
@@ -1230,8 +1242,7 @@ extern "C" int pvlbm_set_streaming(pvlbm_handle h, int mode) {
             fail(PVLBM_EINVAL, "unknown streaming scheme %d", mode);
         if (mode == PVLBM_STREAMING_IN_PLACE && h->wall_mode == PVLBM_WALL_INTERPOLATED)
             fail(PVLBM_EINVAL, "the interpolated wall closure needs two lattices");
-        h->streaming = mode;
-        h->sync_layout();
+        h->set_mode_and_sync(h->streaming, mode);
     });
 }
 
