@@ -17,9 +17,9 @@
 // one extra column in x (to a multiple of 64) and one extra row in y, so that no out-of-box
 // neighbour position aliases a node.  A tile descriptor carries three 64-bit lane masks (lattice
 // node, next-to-wall, boundary) and the first side-table slot of each kind, so a lane finds its
-// slot with mbcnt and no per-lane lookup.  Each pull is one contiguous 512-byte (FP64) / 128-byte
-// (16-bit) run of a direction plane shifted by the stencil offset; every plane has a guard before
-// and after so that all pull addresses are in bounds.  An address is a 64-bit wave-uniform base
+// slot with mbcnt and no per-lane lookup.  Each gather and each store is one contiguous 512-byte
+// (FP64) / 128-byte (16-bit) run of a direction plane, shifted by the stencil offset or not; every
+// plane has a guard before and after so that all addresses are in bounds.  An address is a 64-bit wave-uniform base
 // (plane, stencil offset and tile) plus the lane's byte offset within the tile, so one VGPR serves
 // every direction and a plane may exceed 4 GiB.
 
@@ -137,8 +137,8 @@ __device__ __forceinline__ void save_stale(const StepParams& p, int64_t tb, uint
 // Interpolated bounce-back on the wall links of a wall node (Bouzidi, Firdaouss and Lallemand
 // 2001, linear): for each population D with no upstream node, I = opp D points at the wall and
 // q = bq[slot][I] is where the link meets it.  f*(x) is this node's post-collision population of
-// the previous launch, read from its own slot of A (pull scheme only); f*_I(x - c_I) is the population it pulled in
-// direction I (valid when x - c_I, i.e. x + c_D, is a node).
+// the previous launch, read from its own slot of A (pull scheme only); f*_I(x - c_I) is the
+// population it pulled in direction I (valid when x - c_I, i.e. x + c_D, is a node).
 template <int Q, class ST, int D = 0>
 __device__ __forceinline__ void wall_interpolated(const StepParams& p, int64_t tb, uint32_t lo, uint32_t nmiss,
                                                   const float* bq, typename ST::R (&f)[Q]) {
