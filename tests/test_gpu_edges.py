@@ -122,7 +122,9 @@ def test_store_policy_rule_and_identical_results():
     next launch reads outgrows the last-level cache: in place when the lattice's lines that hold
     nodes (Q S n for an all-fluid box of whole tiles) exceed 240 MiB, with two lattices when one
     step's traffic 2 Q S n exceeds 128 MiB; plain stores below.  Forcing either policy, in either
-    scheme, gives the same fields bit for bit.  A 128 x 128 x 104 all-fluid box, D3Q19: one
+    scheme, gives the same fields bit for bit, and the same FP64 global sums (in place, plain
+    stores sweep every other launch in reverse block order; each block keeps its four tiles, so
+    its partial sums do not change).  A 128 x 128 x 104 all-fluid box, D3Q19: one
     lattice of 259 MB in FP64 (nontemporal in both schemes) and 65 MB in FP16 (plain in both)."""
     from pv_dec_original_amd.simulation import Simulation, SubDomain
     case = fluid_box_case(q=19, box=(128, 128, 104), tau=0.7)
@@ -141,11 +143,12 @@ def test_store_policy_rule_and_identical_results():
                 sim.set_store_policy(policy)
                 expect = {N.STORES_AUTO: auto, N.STORES_REGULAR: False, N.STORES_STREAMING: True}[policy]
                 assert sim.streaming_stores() == expect, (storage, scheme, policy)
-                sim.step(40)
-                fields[(storage, scheme, policy)] = sim.macros()
+                sim.step(41)
+                fields[(storage, scheme, policy)] = sim.macros() + (np.asarray(sim.reduce()),)
                 del sim, sd
         a = fields[(storage, N.STREAMING_IN_PLACE, N.STORES_AUTO)]
         for key, b in fields.items():
             if key[0] == storage:
                 assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), key
+                assert np.array_equal(a[2], b[2]), (key, a[2], b[2])
     assert np.abs(fields[("fp64", N.STREAMING_IN_PLACE, N.STORES_AUTO)][1]).max() > 1e-3
