@@ -152,7 +152,8 @@ int usage() {
                  "               [--storage fp64|fp16|bf16] [--tau T] [--dt DT] [--faces pv|x]\n"
                  "               [--face role,closure,axis,plane,tol,lo0,lo1,hi0,hi1,rho]...\n"
                  "               [--t-end T] [--period P] [--steps N --inlet const:V|sin:M,A,P]\n"
-                 "               [--output-every K] [--outdir DIR] [--vtu] [--wall reference|interpolated]\n";
+                 "               [--output-every K] [--outdir DIR] [--vtu] [--wall reference|interpolated]\n"
+                 "               [--streaming auto|two-lattices|in-place]\n";
     return 2;
 }
 
@@ -164,6 +165,7 @@ int main(int argc, char** argv) {
     int lattice = 19, output_every = 100;
     bool vtu = false;
     std::string wall = "reference";
+    std::string streaming = "auto";
     long steps = -1;
     std::vector<double> grid;
     std::vector<pvlbm_face> faces;
@@ -194,6 +196,7 @@ int main(int argc, char** argv) {
             else if (k == "--outdir") outdir = val();
             else if (k == "--vtu") vtu = true;
             else if (k == "--wall") wall = val();
+            else if (k == "--streaming") streaming = val();
             else if (k == "--help" || k == "-h") return usage(), 0;
             else return usage();
         } catch (const std::exception& e) {
@@ -235,6 +238,9 @@ int main(int argc, char** argv) {
         pvlbm::Simulation sim(sd, tau, 1.0, dt);
         if (wall == "interpolated") sim.setWallClosure(PVLBM_WALL_INTERPOLATED);
         else if (wall != "reference") throw std::runtime_error("--wall takes reference or interpolated");
+        if (streaming == "two-lattices") sim.setStreaming(PVLBM_STREAMING_TWO_LATTICES);
+        else if (streaming == "in-place") sim.setStreaming(PVLBM_STREAMING_IN_PLACE);
+        else if (streaming != "auto") throw std::runtime_error("--streaming takes auto, two-lattices or in-place");
         std::cerr << "grid " << geo->grid.dims[0] << "x" << geo->grid.dims[1] << "x" << geo->grid.dims[2]
                   << ", nodes " << sd.count(PVLBM_LIST_FLUID) << ", inlet " << sd.count(PVLBM_LIST_INLET)
                   << ", wall " << sd.count(PVLBM_LIST_WALL) << ", next-to-wall " << sd.count(PVLBM_LIST_NTW) << "\n";

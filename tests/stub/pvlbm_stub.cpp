@@ -47,6 +47,7 @@ int pvlbm_create(const uint8_t*, const pvlbm_grid*, double, int, int, const pvlb
 int pvlbm_set_tau(pvlbm_handle, double) { return 0; }
 int pvlbm_set_dt(pvlbm_handle, double) { return 0; }
 int pvlbm_set_wall_closure(pvlbm_handle, int) { return 0; }
+int pvlbm_set_streaming(pvlbm_handle, int) { return 0; }
 int pvlbm_count(pvlbm_handle, int, int64_t* n) {
     *n = 1;
     return 0;
