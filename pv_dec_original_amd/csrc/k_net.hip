@@ -1090,17 +1090,19 @@ int pvnet_conv_fwd(const pvnet_conv* c, void* stream) {
     // grid still has >= 2048 blocks (the K loop is long, so the device needs many blocks in flight)
     const int mt = blocks_for(M, 128);
     // brick kernel: 16 NT channels per block, NT = cout / 16 up to 4 (48 and 96: NT = 3); bricks of
-    // 16 x 8 x 4 voxels where their halo fits 64 KB of LDS, else 16 x 4 x 4
+    // 16 x 8 x 4 voxels where their halo fits LIM of LDS, else 16 x 4 x 4
     const size_t tab_bytes = (size_t)(c->kpad / BK) * 4 * sizeof(int);
     const size_t halo8 = (size_t)hl_vox<8>() * c->cin * 2 + tab_bytes, halo4 = (size_t)hl_vox<4>() * c->cin * 2 + tab_bytes;
     const int bnt = c->cout % 16 ? 0 : c->cout <= 64 ? c->cout / 16 : c->cout % 48 == 0 ? 3 : c->cout % 64 == 0 ? 4 : 0;
     const bool bok = bnt && c->ntaps == 27 && !(c->flags & PVNET_GATHER) && c->s.d % BRZ == 0 && c->s.w % BRX == 0 &&
                      M >= (int64_t)256 * 2048;
     // the first of (16 x 8 x 4, weights in LDS), (16 x 8 x 4), (16 x 4 x 4, weights in LDS),
-    // (16 x 4 x 4) whose LDS fits 64 KB: the taller brick first (more waves per CU for the same
-    // LDS), then the weights in LDS (no global load in the k-loop)
+    // (16 x 4 x 4) whose LDS fits LIM: the taller brick first (more waves per CU for the same
+    // LDS), then the weights in LDS (no global load in the k-loop).  A workgroup may take all
+    // 160 KB of a CU's LDS; LIM = 112 KB measured best on the warm-start network's layers (taller
+    // bricks pay even at one block per CU, weights in LDS only while about two blocks fit)
     const size_t wbytes = (size_t)16 * bnt * (2 * (size_t)c->kpad + 16);
-    const size_t LIM = 64 * 1024;
+    const size_t LIM = 112 * 1024;
     bool by8 = false, wl = false;
     size_t lds = 0;
     if (bok && c->s.h % 8 == 0 && halo8 + wbytes <= LIM) {
