@@ -39,7 +39,9 @@ enum { PVNET_RELU = 1, PVNET_F32OUT = 2, PVNET_GATHER = 4 };
  * 3x3x3 convolutions with 16, 32, 48 or 64 outputs (or multiples of 48 or 64) on grids of whole
  * 16 x 4 x 4 bricks (x, y, z; 16 x 8 x 4 where the halo fits) stage each brick and its halo in
  * LDS once; PVNET_GATHER keeps them on the per-tap gathering kernel (the same result bit for bit,
- * for comparison). */
+ * for comparison).  Pointwise convolutions with cin <= 32 and cout <= 64 take their matrix-core
+ * operands straight from global memory (no LDS); PVNET_GATHER keeps them on the tiled kernel
+ * (again the same result bit for bit). */
 typedef struct {
     pvnet_shape s;
     int32_t ntaps, cin, cout, kpad;

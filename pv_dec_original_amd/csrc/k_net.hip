@@ -191,6 +191,59 @@ __global__ __launch_bounds__(256) void k_conv_fwd(pvnet_conv a, int M, int K) {
 }
 
 // ---------------------------------------------------------------------------------------------
+// 1x1x1 convolutions with K = cin <= 32, one MFMA k-step: every wave takes PWT tiles of 16
+// voxels and feeds the matrix core straight from global memory, with no LDS and no barrier.  Lane
+// (c16, g) holds the input channels 8g .. 8g+7 of voxel c16 of a tile (zero beyond cin) and the
+// weight row 16 j + c16 at the same k-values: the operands k_conv_fwd stages for the same
+// convolution, so the outputs are the same bit for bit.
+constexpr int PWT = 4;
+
+template <int NJ>
+__global__ __launch_bounds__(256) void k_conv_pw(pvnet_conv a, int M) {
+    const int lane = threadIdx.x & 63, g = lane >> 4, c16 = lane & 15;
+    const int64_t tile0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * PWT;
+    bf16x8 bfr[NJ];
+    float bias[NJ];
+#pragma unroll
+    for (int j = 0; j < NJ; j++) {
+        const int n = 16 * j + c16;
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (n < a.cout) v = *(const uint4*)(a.w + (int64_t)n * a.kpad + g * 8);
+        bfr[j] = *reinterpret_cast<const bf16x8*>(&v);
+        bias[j] = (n < a.cout && a.bias) ? a.bias[n] : 0.f;
+    }
+    uint4 ra[PWT];
+#pragma unroll
+    for (int i = 0; i < PWT; i++) {
+        const int64_t m = (tile0 + i) * 16 + c16;
+        ra[i] = make_uint4(0, 0, 0, 0);
+        if (m < M && g * 8 < a.cin) ra[i] = *(const uint4*)(a.x + m * a.ldx + g * 8);
+    }
+    const bool relu = a.flags & PVNET_RELU, f32out = a.flags & PVNET_F32OUT;
+#pragma unroll
+    for (int i = 0; i < PWT; i++)
+#pragma unroll
+        for (int j = 0; j < NJ; j++) {
+            const f32x4 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(*reinterpret_cast<const bf16x8*>(&ra[i]), bfr[j],
+                                                                      f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+            const int n = 16 * j + c16;
+            if (n >= a.cout) continue;
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int64_t m = (tile0 + i) * 16 + g * 4 + e;  // MFMA row 4 g + e
+                if (m >= M) continue;
+                float v = acc[e] + bias[j];
+                if (relu) v = v < 0.f ? 0.f : v;
+                if (a.relu_of && !bf_pos(a.relu_of[m * a.ldr + n])) v = 0.f;
+                if (f32out)
+                    ((float*)a.y)[m * a.ldy + n] = v;
+                else
+                    ((uint16_t*)a.y)[m * a.ldy + n] = f2bf(v);
+            }
+        }
+}
+
+// ---------------------------------------------------------------------------------------------
 // 3x3x3 convolution with 16 NT output channels per block (NT = 1..4 column tiles; grid.y takes
 // further groups of 16 NT channels) on grids of whole 16 x 4 x 4
 // bricks: each block stages one brick of one box with its one-voxel halo (18 x 6 x 6 voxels, all
@@ -1143,6 +1196,14 @@ int pvnet_conv_fwd(const pvnet_conv* c, void* stream) {
             }
         }
 #undef PVNET_BRICK
+    } else if (c->ntaps == 1 && K <= BK && c->cout <= 64 && !(c->flags & PVNET_GATHER)) {
+        const dim3 gp((unsigned)((M + 64 * PWT - 1) / (64 * PWT)));
+        switch ((c->cout + 15) / 16) {
+            case 1: hipLaunchKernelGGL((k_conv_pw<1>), gp, dim3(256), 0, st, *c, (int)M); break;
+            case 2: hipLaunchKernelGGL((k_conv_pw<2>), gp, dim3(256), 0, st, *c, (int)M); break;
+            case 3: hipLaunchKernelGGL((k_conv_pw<3>), gp, dim3(256), 0, st, *c, (int)M); break;
+            default: hipLaunchKernelGGL((k_conv_pw<4>), gp, dim3(256), 0, st, *c, (int)M); break;
+        }
     } else if (c->cout <= 16 && M >= (int64_t)256 * 2048) {
         hipLaunchKernelGGL((k_conv_fwd<16, 64>), dim3(blocks_for(M, 256), 1), dim3(256), 0, st, *c, (int)M, K);
     } else if (c->cout <= 16) {

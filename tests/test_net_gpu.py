@@ -206,6 +206,43 @@ def test_conv_fwd_brick_equals_gather(T, cin, cout, flags, ldx, xoff, ldy, yoff,
     assert a[:, yoff:yoff + cout].abs().max().item() > 0
 
 
+@pytest.mark.parametrize("shape,cin,cout,flags,ldx,xoff,ldy,yoff,mask", [
+    ((2, 8, 16, 24), 16, 8, 2, 16, 0, 8, 0, False), ((2, 8, 16, 24), 8, 16, 0, 8, 0, 16, 0, True),
+    ((1, 32, 96, 96), 16, 8, 2, 24, 8, 8, 0, False), ((1, 32, 96, 96), 8, 16, 0, 8, 0, 16, 0, True),
+    ((1, 12, 20, 28), 32, 40, 1, 48, 16, 64, 8, False), ((3, 4, 4, 5), 24, 64, 1, 24, 0, 64, 0, True)])
+def test_conv_pointwise_equals_gather(T, shape, cin, cout, flags, ldx, xoff, ldy, yoff, mask):
+    """1x1x1 convolutions with K <= 32 feed the matrix core from global memory (no LDS); the result
+    must equal the tiled kernel's (PVNET_GATHER) bit for bit, with input and output channel slices,
+    ReLU, FP32 output and the ReLU-backward mask, including voxel counts that leave a partial
+    tile.  Columns outside the output slice stay untouched."""
+    from pv_dec_original_amd import _native as N
+    NL = N.net()
+    M = O.nvox(shape)
+    xfull = np.zeros((M, ldx))
+    xfull[:, xoff:xoff + cin] = _rbf((M, cin))
+    w = O.bf16_round(_rbf((cout, cin)) / np.sqrt(cin))
+    b = RNG.standard_normal(cout).astype(np.float32).astype(np.float64)
+    xd, w32, bd = _dev(T, xfull), _dev(T, w, T.float32), _dev(T, b, T.float32)
+    kpad = (cin + 31) // 32 * 32
+    wt = T.zeros((cout, kpad), dtype=T.bfloat16, device="cuda")
+    wdp = T.zeros((cin, (cout + 31) // 32 * 32), dtype=T.bfloat16, device="cuda")
+    N.net_check(NL.pvnet_pack_weights(_p(w32), 1, cin, cout, _p(wt), _p(wdp), _st(T)))
+    rel = _dev(T, _rbf((M, cout))) if mask else None
+    f32 = bool(flags & 2)
+    out = {}
+    for gather in (False, True):
+        yd = T.full((M, ldy), 7.0, dtype=T.float32 if f32 else T.bfloat16, device="cuda")
+        c = N.NetConv(N.NetShape(*shape), 1, cin, cout, kpad, _p(xd, xoff), ldx, flags | (N.NET_GATHER if gather else 0),
+                      _p(wt), _p(bd), _p(yd, yoff), ldy, cout if mask else 0, _p(rel) if mask else None)
+        N.net_check(NL.pvnet_conv_fwd(C.byref(c), _st(T)))
+        T.cuda.synchronize()
+        out[gather] = yd
+    a, g = out[False], out[True]
+    assert T.equal(a, g), f"{(a != g).sum().item()} of {a.numel()} differ"
+    assert (a[:, :yoff] == 7.0).all() and (a[:, yoff + cout:] == 7.0).all()
+    assert a[:, yoff:yoff + cout].abs().max().item() > 0
+
+
 @pytest.mark.parametrize("shape,cin,cout,ldx,xoff", [
     ((2, 8, 16, 16), 8, 16, 8, 0), ((1, 12, 16, 24), 16, 16, 24, 8), ((3, 4, 8, 8), 24, 16, 24, 0),
     ((2, 8, 8, 16), 48, 16, 56, 8), ((2, 8, 16, 8), 8, 32, 8, 0), ((1, 8, 8, 16), 32, 32, 48, 16),
