@@ -711,20 +711,36 @@ __global__ __launch_bounds__(256) void k_colsum(const uint16_t* dy, int ld, int 
 // sum the rows rg, rg + 16, ... in ascending order, and the 16 partials are added in ascending rg
 constexpr int SUM_COLS = 16;
 
-__global__ __launch_bounds__(256) void k_sum_splits(const float* src, int nsplit, int64_t len, float* dst,
-                                                    int64_t ldd, int64_t rowlen) {
+// One reduction of split partials: dst[(i / rowlen) ldd + i % rowlen] = sum_s src[s len + i].
+struct SumJob {
+    const float* src;
+    float* dst;
+    int64_t len, ldd, rowlen;
+    int nsplit;
+    int blocks;  // blocks_for(len, SUM_COLS)
+};
+
+// Two reductions in one launch (the weight and bias gradients of a layer): blocks below
+// a.blocks take job a, the others job b, each summing exactly as alone.
+__global__ __launch_bounds__(256) void k_sum_splits(const SumJob a, const SumJob b) {
     __shared__ float sp[16][SUM_COLS + 1];
+    const bool second = (int)blockIdx.x >= a.blocks;
+    const float* src = second ? b.src : a.src;
+    float* dst = second ? b.dst : a.dst;
+    const int64_t len = second ? b.len : a.len, ldd = second ? b.ldd : a.ldd, rowlen = second ? b.rowlen : a.rowlen;
+    const int nsplit = second ? b.nsplit : a.nsplit;
+    const int blk = second ? (int)blockIdx.x - a.blocks : (int)blockIdx.x;
     const int t = threadIdx.x, col = t % SUM_COLS, rg = t / SUM_COLS;
-    const int64_t i = (int64_t)blockIdx.x * SUM_COLS + col;
+    const int64_t i = (int64_t)blk * SUM_COLS + col;
     float s = 0.f;
     if (i < len)
         for (int p = rg; p < nsplit; p += 16) s += src[(int64_t)p * len + i];
     sp[rg][col] = s;
     __syncthreads();
     if (t < SUM_COLS && i < len) {
-        float a = 0.f;
-        for (int r = 0; r < 16; r++) a += sp[r][t];
-        dst[(i / rowlen) * ldd + i % rowlen] = a;
+        float acc = 0.f;
+        for (int r = 0; r < 16; r++) acc += sp[r][t];
+        dst[(i / rowlen) * ldd + i % rowlen] = acc;
     }
 }
 
@@ -1263,18 +1279,20 @@ int pvnet_conv_wgrad(const pvnet_wgrad* a, void* stream) {
         hipLaunchKernelGGL(k_conv_wgrad<64>, grid, dim3(256), 0, st, *a, (int)M, K, p.mchunk);
     NET_LAUNCH();
     const int64_t len = (int64_t)a->cout * K;
-    hipLaunchKernelGGL(k_sum_splits, dim3(blocks_for(len, SUM_COLS)), dim3(256), 0, st, a->ws, nsplit, len, a->dw,
-                       (int64_t)K, (int64_t)K);
-    NET_LAUNCH();
+    const SumJob jw{a->ws, a->dw, len, (int64_t)K, (int64_t)K, nsplit, (int)blocks_for(len, SUM_COLS)};
+    SumJob jb{nullptr, nullptr, 0, 1, 1, 0, 0};
     if (a->db) {
+        // the bias column sums read only dy: they run before the split sums, whose launch then
+        // reduces the weight and the bias partials together
         if (a->cout > 256 || a->cout % 8) return fail(1, "pvnet_conv_wgrad: bias gradient needs cout <= 256, a multiple of 8");
         float* part = a->ws + p.ws_main;
         hipLaunchKernelGGL(k_colsum, dim3(p.cblk), dim3(256), 0, st, a->dy, a->lddy, a->cout, (int)M, p.cchunk, part);
         NET_LAUNCH();
-        hipLaunchKernelGGL(k_sum_splits, dim3(blocks_for(a->cout, SUM_COLS)), dim3(256), 0, st, part, p.cblk,
-                           (int64_t)a->cout, a->db, (int64_t)a->cout, (int64_t)a->cout);
-        NET_LAUNCH();
+        jb = SumJob{part, a->db, (int64_t)a->cout, (int64_t)a->cout, (int64_t)a->cout, p.cblk,
+                    (int)blocks_for(a->cout, SUM_COLS)};
     }
+    hipLaunchKernelGGL(k_sum_splits, dim3(jw.blocks + jb.blocks), dim3(256), 0, st, jw, jb);
+    NET_LAUNCH();
     return 0;
 }
 
