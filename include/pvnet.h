@@ -59,7 +59,8 @@ typedef struct {
 int pvnet_conv_fwd(const pvnet_conv* c, void* stream);
 
 /* dw[n][k] = sum_m dy[m][n] * x[m + off(tap(k))][ci(k)]  for k < ntaps*cin (row length ntaps*cin)
- * db[n]    = sum_m dy[m][n]                               (if db is not NULL)
+ * db[n]    = sum_m dy[m][n]                               (if db is not NULL; the 16-output brick
+ *                                                          kernels sum it in the same pass)
  * ws: FP32 scratch of at least pvnet_wgrad_ws_floats(...) floats for the split-K partials; the
  * sums are reduced in a fixed order, so results are deterministic.  16- and 32-output 3x3x3 layers
  * on grids of whole 8 x 8 x 4 bricks reduce brick by brick with the input staged in LDS;

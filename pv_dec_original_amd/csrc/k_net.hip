@@ -566,8 +566,13 @@ __device__ __forceinline__ void store_cols(const uint4 (&r)[8], uint4* dst, int 
 // of 16 output channels (cout = 16 NR).  The grid is one-dimensional, ngy blocks per split: the
 // blocks of one split take consecutive slots of one XCD, so that they walk the same bricks at the
 // same time through one L2 (each reads its own channels of the same input rows).
+//
+// dboff >= 0 (16-output layers): the blocks of the first channel group also sum dy over their
+// bricks for the bias gradient, as one more MFMA per fragment with B = ones in their first wave,
+// into the row ws[dboff + split * cout ..] (the split sums reduce those rows like the weights').
 template <int NGB, int NR>
-__global__ __launch_bounds__(256) void k_conv_wgrad_brick(pvnet_wgrad a, int K, int nbricks, int per, int ngy) {
+__global__ __launch_bounds__(256) void k_conv_wgrad_brick(pvnet_wgrad a, int K, int nbricks, int per, int ngy,
+                                                          int64_t dboff) {
     // X ends in WBX_ZERO zero rows: the lanes of the empty half of the last tap pair read there
     __shared__ uint4 X[WBX_ROWS + WBX_ZERO];
     __shared__ uint4 DY[16 * NR * WBD_P];
@@ -587,6 +592,11 @@ __global__ __launch_bounds__(256) void k_conv_wgrad_brick(pvnet_wgrad a, int K, 
         for (int i = 0; i < 4; i++)
 #pragma unroll
             for (int rt = 0; rt < NR; rt++) acc[j][i][rt] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const bool dob = NR == 1 && dboff >= 0 && cg0 == 0 && wv == 0;  // wave-uniform
+    f32x4 accb = f32x4{0.f, 0.f, 0.f, 0.f};
+    bf16x8 ones;
+#pragma unroll
+    for (int e = 0; e < 8; e++) ones[e] = (__bf16)1.0f;
 
     const int cl = c16 & 7;  // this lane's channel within the group; its tap is 2 p + (c16 >> 3)
     if (t < WBX_ZERO) X[WBX_ROWS + t] = make_uint4(0, 0, 0, 0);  // ordered by the first barrier
@@ -638,6 +648,12 @@ __global__ __launch_bounds__(256) void k_conv_wgrad_brick(pvnet_wgrad a, int K, 
 #pragma unroll
                 for (int s2 = 0; s2 < 8; s2++)
                     af[rt][s2] = *reinterpret_cast<const bf16x8*>(&DY[(16 * rt + c16) * WBD_P + 4 * s2 + g]);
+            if (NR == 1 && dob && j == 0) {
+                // every column of dy^T x ones is the sum of dy over the brick's voxels
+#pragma unroll
+                for (int s2 = 0; s2 < 8; s2++)
+                    accb = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[0][s2], ones, accb, 0, 0, 0);
+            }
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 const int p = wv + 4 * i;
@@ -678,6 +694,11 @@ __global__ __launch_bounds__(256) void k_conv_wgrad_brick(pvnet_wgrad a, int K, 
                 for (int e = 0; e < 4; e++)
                     a.ws[((int64_t)split * 16 * NR + 16 * rt + 4 * g + e) * K + k] = acc[j][i][rt][e];
         }
+    }
+    // D[4 g + e][c16] = sum over this block's voxels of dy[.][4 g + e], the same in every column
+    if (dob && c16 == 0) {
+#pragma unroll
+        for (int e = 0; e < 4; e++) a.ws[dboff + (int64_t)split * 16 + 4 * g + e] = accb[e];
     }
 }
 
@@ -1123,7 +1144,8 @@ WgradPlan plan_wgrad(const pvnet_shape& s, int ntaps, int cin, int cout, int fla
     }
     p.cchunk = (int)std::max<int64_t>(256, ((M + 1023) / 1024 + 255) & ~(int64_t)255);  // ~1024 blocks
     p.cblk = (int)((M + p.cchunk - 1) / p.cchunk);
-    p.ws_total = p.ws_main + (int64_t)p.cblk * cout;
+    // the bias rows: one per column-sum block, or one per split when the brick kernel sums them
+    p.ws_total = p.ws_main + (int64_t)std::max(p.cblk, p.brick ? p.nsplit : 0) * cout;
     return p;
 }
 
@@ -1255,10 +1277,11 @@ int pvnet_conv_wgrad(const pvnet_wgrad* a, void* stream) {
     const int K = a->ntaps * a->cin;
     dim3 grid(p.ktiles, p.ntiles, p.nsplit);
     int nsplit = p.nsplit;
+    bool fused_db = false;
     if (p.brick) {
         // one round of resident blocks along the voxels (at most the plan's WB_SPLITS partial rows)
         const int ng = a->cin / 8;
-        void (*kern)(pvnet_wgrad, int, int, int, int) =
+        void (*kern)(pvnet_wgrad, int, int, int, int, int64_t) =
             a->cout == 32 ? (ng == 1 ? k_conv_wgrad_brick<1, 2> : k_conv_wgrad_brick<WB_NGB, 2>)
                           : (ng == 1 ? k_conv_wgrad_brick<1, 1> : k_conv_wgrad_brick<WB_NGB, 1>);
         int dev = 0, cus = 0, occ = 0;
@@ -1270,7 +1293,10 @@ int pvnet_conv_wgrad(const pvnet_wgrad* a, void* stream) {
         const int want = std::max(1, std::min({p.nbricks, WB_SPLITS, std::max(1, cus * occ / gy)}));
         const int per = (p.nbricks + want - 1) / want;
         nsplit = (p.nbricks + per - 1) / per;
-        hipLaunchKernelGGL(kern, dim3(nsplit * gy), dim3(256), 0, st, *a, K, p.nbricks, per, gy);
+        // 16-output layers sum the bias gradient in the same pass (rows after the weight partials)
+        fused_db = a->db && a->cout == 16;
+        hipLaunchKernelGGL(kern, dim3(nsplit * gy), dim3(256), 0, st, *a, K, p.nbricks, per, gy,
+                           fused_db ? p.ws_main : (int64_t)-1);
     } else if (p.bn == 16)
         hipLaunchKernelGGL(k_conv_wgrad<16>, grid, dim3(256), 0, st, *a, (int)M, K, p.mchunk);
     else if (p.bn == 32)
@@ -1281,7 +1307,10 @@ int pvnet_conv_wgrad(const pvnet_wgrad* a, void* stream) {
     const int64_t len = (int64_t)a->cout * K;
     const SumJob jw{a->ws, a->dw, len, (int64_t)K, (int64_t)K, nsplit, (int)blocks_for(len, SUM_COLS)};
     SumJob jb{nullptr, nullptr, 0, 1, 1, 0, 0};
-    if (a->db) {
+    if (fused_db) {
+        jb = SumJob{a->ws + p.ws_main, a->db, (int64_t)a->cout, (int64_t)a->cout, (int64_t)a->cout, nsplit,
+                    (int)blocks_for(a->cout, SUM_COLS)};
+    } else if (a->db) {
         // the bias column sums read only dy: they run before the split sums, whose launch then
         // reduces the weight and the bias partials together
         if (a->cout > 256 || a->cout % 8) return fail(1, "pvnet_conv_wgrad: bias gradient needs cout <= 256, a multiple of 8");

@@ -252,7 +252,8 @@ def test_wgrad_brick_against_oracle_and_gather(T, shape, cin, cout, ldx, xoff):
     brick with the input staged in LDS: against the FP64 oracle (FP32 accumulation, 1e-5 of the
     scale) and against the per-tap gathering kernel (PVNET_GATHER, the same sums in another order),
     over several boxes (zero padding at every face), one to twelve 8-channel groups, and an input
-    slice of a wider tensor; two runs give the same bits."""
+    slice of a wider tensor; two runs give the same bits.  The bias gradient too: the 16-output
+    brick kernels sum it in the same pass (dy^T times ones on the matrix core)."""
     from pv_dec_original_amd import _native as N
     NL = N.net()
     M = O.nvox(shape)
@@ -273,9 +274,11 @@ def test_wgrad_brick_against_oracle_and_gather(T, shape, cin, cout, ldx, xoff):
         N.net_check(NL.pvnet_conv_wgrad(C.byref(g), _st(T)))
         T.cuda.synchronize()
         out[run] = _np(dw)
+        out[run + " db"] = _np(db)
         assert np.allclose(out[run], refdw, rtol=1e-5, atol=1e-5 * sc), (run, np.abs(out[run] - refdw).max() / sc)
-        assert np.allclose(_np(db), refdb, rtol=1e-5, atol=1e-5 * np.abs(refdb).max())
+        assert np.allclose(_np(db), refdb, rtol=1e-5, atol=1e-5 * np.abs(refdb).max()), run
     assert np.array_equal(out["brick"], out["brick again"])
+    assert np.array_equal(out["brick db"], out["brick again db"])
     assert np.allclose(out["brick"], out["gather"], rtol=1e-5, atol=1e-6 * sc)
 
 
