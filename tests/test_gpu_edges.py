@@ -98,6 +98,9 @@ def test_bad_arguments_are_errors_and_leave_the_state():
     assert L.pvlbm_set_wall_closure(h, 7) == EINVAL
     assert L.pvlbm_set_wall_closure(h, N.WALL_INTERPOLATED) == ESTATE and "triangles" in msg()
     assert L.pvlbm_set_tau(h, 0.5) == EINVAL and L.pvlbm_set_wss_every(h, 0) == EINVAL
+    assert L.pvlbm_set_streaming(h, 7) == EINVAL and "streaming" in msg()
+    assert L.pvlbm_get_streaming(h, None) == EINVAL
+    assert L.pvlbm_set_store_policy(h, 9) == EINVAL
     assert L.pvlbm_step(h, -1, None) == EINVAL
     import torch
     rc = torch.ones(8, dtype=torch.float32, device="cuda")  # valid device fields of a 2x2x2 grid
