@@ -6,7 +6,8 @@ windows (ComplexGeometry.cpp:1376-1408) in the precedence inlet, outlet 1, 2, 3
 (Subdomain.cpp:124-155).  workloads.branch_case builds that arrangement on a generated vessel,
 with windows that cut an opening in two and overlap (see its docstring).  Voxel flags, node kinds
 and every list must equal the oracle's; rho, u, WSS, its vector and the accumulators must agree
-bit for bit over a pulsatile run.
+bit for bit over a pulsatile run, streaming in place (the default at this size) and again with
+two lattices.
 """
 import numpy as np
 import pytest
@@ -72,6 +73,19 @@ def test_branch_outlets_ymax_zmin_bit_exact():
     assert vy.mean() > 0 and vz.mean() < 0  # leaving through +y and through -z
     assert sim.gate_events() == 0 and orc.gate_events() == 0
     assert np.isfinite(rho).all() and np.isfinite(u).all()
+    # the same run with two lattices: the same bits
+    from pv_dec_original_amd.simulation import Simulation, SubDomain
+    assert sim.in_place
+    sd2 = SubDomain(geo, case.faces, storage="fp64", tau=case.tau, dt=G.DT)
+    sim2 = Simulation(sd2, case.tau, 1.0, G.DT)
+    sim2.set_inlet_profile(case.inlet_profile(sd2, 0.05))
+    sim2.set_streaming(N.STREAMING_TWO_LATTICES)
+    sim2.step(steps, scale)
+    rho2, u2 = sim2.macros()
+    assert np.array_equal(rho2, rho) and np.array_equal(u2, u)
+    assert np.array_equal(sim2.wss(), w, equal_nan=True)
+    tau2, acc2 = sim2.wss_raw()
+    assert np.array_equal(tau2, tau_g) and np.array_equal(acc2, acc_g)
 
 
 def test_face_plane_low_part_on_device():

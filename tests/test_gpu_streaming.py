@@ -29,7 +29,8 @@ def _same(a, b):
     return all(np.array_equal(x, y, equal_nan=True) for x, y in zip(a, b))
 
 
-@pytest.mark.parametrize("q,storage", [(19, "fp64"), (27, "fp64"), (19, "fp16"), (27, "bf16")])
+@pytest.mark.parametrize("q,storage", [(19, "fp64"), (27, "fp64"), (19, "fp16"), (27, "fp16"), (19, "bf16"),
+                                       (27, "bf16")])
 def test_in_place_equals_two_lattices(q, storage):
     case = G.small_ybif_case(q=q)
     sd_a, sim_a, _ = G.build(case, storage, normals=False)
