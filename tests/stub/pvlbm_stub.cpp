@@ -1,5 +1,5 @@
 // A stand-in for libpvlbm.so for the CPU test of lbm_run's loop (tests/test_cli.py): it records the
-// order of the calls the driver makes -- steps, macro reads, WSS reads -- on stderr and returns
+// order of the calls lbm_run makes -- steps, macro reads, WSS reads -- on stderr and returns
 // small fixed data.  It computes nothing and is never shipped or loaded by the package.
 #include <cstdio>
 #include <cstring>
