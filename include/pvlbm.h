@@ -279,13 +279,18 @@ int pvlbm_get_streaming(pvlbm_handle h, int* in_place);
  * (240 MiB); with two lattices, when one step's traffic, 2 * Q * bytes per population * nodes,
  * exceeds PVLBM_STORES_AUTO_BYTES (128 MiB).  In place with plain stores, every other launch
  * sweeps the lattice in reverse, so that it starts on the lines the previous launch touched last.
- * PVLBM_STORES_REGULAR / _STREAMING force one.  pvlbm_get_store_policy reports the stores in use
- * (1 = nontemporal). */
+ * Under PVLBM_STORES_AUTO the in-place step with plain stores also reads the populations with
+ * nontemporal loads once those lines exceed PVLBM_LOADS_AUTO_INPLACE_BYTES (180 MiB), a lattice
+ * that nearly fills the cache.  PVLBM_STORES_REGULAR / _STREAMING force one kind of store, with
+ * plain loads.  pvlbm_get_store_policy reports the stores in use (1 = nontemporal),
+ * pvlbm_get_load_policy the loads (1 = nontemporal). */
 enum { PVLBM_STORES_AUTO = 0, PVLBM_STORES_REGULAR = 1, PVLBM_STORES_STREAMING = 2 };
 #define PVLBM_STORES_AUTO_BYTES (128ll << 20)
 #define PVLBM_STORES_AUTO_INPLACE_BYTES (240ll << 20)
+#define PVLBM_LOADS_AUTO_INPLACE_BYTES (180ll << 20)
 int pvlbm_set_store_policy(pvlbm_handle h, int policy);
 int pvlbm_get_store_policy(pvlbm_handle h, int* streaming);
+int pvlbm_get_load_policy(pvlbm_handle h, int* streaming);
 
 /* Every how many iterations WSS and its accumulators are evaluated (reference: every iteration,
  * Simulation.cpp:585-586).  Default 1, the reference's quantities.  With every > 1 the accumulators

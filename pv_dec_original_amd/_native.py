@@ -105,6 +105,7 @@ PVLBM_API = [
     ("pvlbm_set_wall_closure", C.c_int, [C.c_void_p, C.c_int]),
     ("pvlbm_set_store_policy", C.c_int, [C.c_void_p, C.c_int]),
     ("pvlbm_get_store_policy", C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    ("pvlbm_get_load_policy", C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     ("pvlbm_set_streaming", C.c_int, [C.c_void_p, C.c_int]),
     ("pvlbm_get_streaming", C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     ("pvlbm_step", C.c_int, [C.c_void_p, C.c_int, _d]),

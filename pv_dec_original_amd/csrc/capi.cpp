@@ -159,6 +159,12 @@ struct pvlbm_sim {
         if (in_place()) return footprint > PVLBM_STORES_AUTO_INPLACE_BYTES;
         return (int64_t)(2 * q * store_bytes()) * nfluid > PVLBM_STORES_AUTO_BYTES;
     }
+    // nontemporal population loads: under the automatic policy, in place with plain stores, on a
+    // lattice whose lines holding nodes nearly fill the last-level cache
+    bool stream_loads() const {
+        return store_policy == PVLBM_STORES_AUTO && in_place() && !stream_stores() &&
+               footprint > PVLBM_LOADS_AUTO_INPLACE_BYTES;
+    }
     size_t real_bytes() const { return storage == PVLBM_FP64 ? 8 : 4; }
     // in place on one lattice when asked, or by default while that lattice is within the cache
     // budget; never under the interpolated closure, which reads the node's previous populations
@@ -212,6 +218,7 @@ struct pvlbm_sim {
         p.blocks = blocks();
         p.flags = flags;
         p.stream_stores = stream_stores() ? 1 : 0;
+        p.stream_loads = stream_loads() ? 1 : 0;
         p.bmask = d_bmask;
         p.bclose = d_bclose;
         p.bprof = d_bprof;
@@ -1267,6 +1274,14 @@ extern "C" int pvlbm_get_store_policy(pvlbm_handle h, int* streaming) {
     return guarded(h, [&] {
         if (!streaming) fail(PVLBM_EINVAL, "streaming is NULL");
         *streaming = h->stream_stores() ? 1 : 0;
+    });
+}
+
+extern "C" int pvlbm_get_load_policy(pvlbm_handle h, int* streaming) {
+    if (!h) return PVLBM_EINVAL;
+    return guarded(h, [&] {
+        if (!streaming) fail(PVLBM_EINVAL, "streaming is NULL");
+        *streaming = h->stream_loads() ? 1 : 0;
     });
 }
 

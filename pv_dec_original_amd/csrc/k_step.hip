@@ -108,12 +108,15 @@ __device__ __forceinline__ S* at(P* base, int64_t sbytes, uint32_t lo) {
 }
 
 // All Q gathers issued back to back, before any store of the lane (in the in-place scheme a lane
-// writes the slots it read).
-template <int Q, class ST, int SCH, int D = 0>
+// writes the slots it read).  NL: nontemporal loads (in place, for a lattice that nearly fills the
+// last-level cache).
+template <int Q, class ST, int SCH, bool NL, int D = 0>
 __device__ __forceinline__ void load_all(const StepParams& p, int64_t tb, uint32_t lo, typename ST::S (&v)[Q]) {
     if constexpr (D < Q) {
-        v[D] = *at<const typename ST::S>(p.A, load_off<Q, SCH, D>(p) + tb, lo);
-        load_all<Q, ST, SCH, D + 1>(p, tb, lo, v);
+        const typename ST::S* src = at<const typename ST::S>(p.A, load_off<Q, SCH, D>(p) + tb, lo);
+        if constexpr (NL) v[D] = __builtin_nontemporal_load(src);
+        else v[D] = *src;
+        load_all<Q, ST, SCH, NL, D + 1>(p, tb, lo, v);
     }
 }
 
@@ -226,7 +229,7 @@ __device__ __forceinline__ void collide_store_wss(const StepParams& p, int64_t t
 // reference closure does not carry its registers); it reads the node's previous populations from
 // its own slots, which exist only in the pull scheme.  SCH: the kind of launch (enum Scheme); an
 // F_INIT launch of the in-place scheme is an even one without the gathers.
-template <int Q, class ST, bool WSS, bool NT, bool WI, int SCH>
+template <int Q, class ST, bool WSS, bool NT, bool WI, int SCH, bool NL>
 __device__ __forceinline__ bool node_update(const StepParams& p, int blk, double (&red)[4]) {
     static_assert(!WI || SCH == SCH_PULL, "the interpolated closure needs the pull scheme");
     using R = typename ST::R;
@@ -282,7 +285,7 @@ __device__ __forceinline__ bool node_update(const StepParams& p, int blk, double
     R f[Q];
     if (!init) {
         S v[Q];
-        load_all<Q, ST, SCH>(p, tb, lo, v);
+        load_all<Q, ST, SCH, NL>(p, tb, lo, v);
 #pragma unroll
         for (int d = 0; d < Q; d++) f[d] = ST::load(v[d], Lat<Q>::w[d]);
         if constexpr (ST::kGate) {
@@ -382,11 +385,11 @@ __device__ __forceinline__ bool node_update(const StepParams& p, int blk, double
 // Block b takes tiles 4b .. 4b+3, or with p.reverse block b takes the tiles of block
 // blocks-1-b: workgroups start in index order, so the sweep runs from the other end of the
 // lattice, while each group of four tiles (and so each block partial sum) stays the same.
-template <int Q, class ST, bool WSS, bool NT, bool WI, int SCH>
+template <int Q, class ST, bool WSS, bool NT, bool WI, int SCH, bool NL>
 __global__ __launch_bounds__(256) void k_step(const StepParams p) {
     double red[4] = {0.0, 0.0, 0.0, 0.0};
     const int blk = p.reverse ? p.blocks - 1 - (int)blockIdx.x : (int)blockIdx.x;
-    node_update<Q, ST, WSS, NT, WI, SCH>(p, blk, red);
+    node_update<Q, ST, WSS, NT, WI, SCH, NL>(p, blk, red);
     if (p.flags & F_REDUCE) {
         // FP64 block sums of (mass, inlet flux, outlet flux, non-finite count)
         __shared__ double s[4][4];
@@ -454,11 +457,20 @@ void launch_qws(const StepParams& p, hipStream_t s) {
     // nontemporal stores keep a lattice larger than the caches from evicting the gathers; a
     // per-step working set that fits the last-level cache is served from it with plain stores
     if (p.stream_stores) {
-        if (p.flags & F_WSS) hipLaunchKernelGGL((k_step<Q, ST, true, true, WI, SCH>), g, b, 0, s, p);
-        else hipLaunchKernelGGL((k_step<Q, ST, false, true, WI, SCH>), g, b, 0, s, p);
+        if (p.flags & F_WSS) hipLaunchKernelGGL((k_step<Q, ST, true, true, WI, SCH, false>), g, b, 0, s, p);
+        else hipLaunchKernelGGL((k_step<Q, ST, false, true, WI, SCH, false>), g, b, 0, s, p);
+    } else if constexpr (SCH != SCH_PULL) {
+        // in place on a lattice that nearly fills the last-level cache: nontemporal loads
+        if (p.stream_loads) {
+            if (p.flags & F_WSS) hipLaunchKernelGGL((k_step<Q, ST, true, false, WI, SCH, true>), g, b, 0, s, p);
+            else hipLaunchKernelGGL((k_step<Q, ST, false, false, WI, SCH, true>), g, b, 0, s, p);
+        } else {
+            if (p.flags & F_WSS) hipLaunchKernelGGL((k_step<Q, ST, true, false, WI, SCH, false>), g, b, 0, s, p);
+            else hipLaunchKernelGGL((k_step<Q, ST, false, false, WI, SCH, false>), g, b, 0, s, p);
+        }
     } else {
-        if (p.flags & F_WSS) hipLaunchKernelGGL((k_step<Q, ST, true, false, WI, SCH>), g, b, 0, s, p);
-        else hipLaunchKernelGGL((k_step<Q, ST, false, false, WI, SCH>), g, b, 0, s, p);
+        if (p.flags & F_WSS) hipLaunchKernelGGL((k_step<Q, ST, true, false, WI, SCH, false>), g, b, 0, s, p);
+        else hipLaunchKernelGGL((k_step<Q, ST, false, false, WI, SCH, false>), g, b, 0, s, p);
     }
 }
 

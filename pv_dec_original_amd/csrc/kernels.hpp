@@ -57,6 +57,7 @@ struct StepParams {
     int32_t blocks;         // launched blocks (4 tiles each)
     int32_t flags;
     int32_t stream_stores;  // 1: nontemporal population stores (per-step traffic above the cache)
+    int32_t stream_loads;   // 1: nontemporal population loads (in place with plain stores only)
     int32_t scheme;         // enum Scheme
     int32_t reverse;        // 1: blocks take their tiles from the end (a sweep in the other direction)
     // boundary table (inlet, outlet and wall nodes)

@@ -199,14 +199,20 @@ class Simulation:
     def set_store_policy(self, policy):
         """N.STORES_AUTO (default: nontemporal population stores when the lattice the next launch
         reads exceeds the cache budget: in place its lines holding nodes > 240 MiB, two lattices
-        2*Q*S*n > 128 MiB),
-        N.STORES_REGULAR or N.STORES_STREAMING; results are the same under each."""
+        2*Q*S*n > 128 MiB; in place with plain stores, nontemporal loads above 180 MiB of lines),
+        N.STORES_REGULAR or N.STORES_STREAMING (plain loads); results are the same under each."""
         self._chk(N.lib().pvlbm_set_store_policy(self.handle, int(policy)))
 
     def streaming_stores(self):
         """True when the step writes the populations with nontemporal stores."""
         v = C.c_int(0)
         self._chk(N.lib().pvlbm_get_store_policy(self.handle, C.byref(v)))
+        return bool(v.value)
+
+    def streaming_loads(self):
+        """True when the step reads the populations with nontemporal loads."""
+        v = C.c_int(0)
+        self._chk(N.lib().pvlbm_get_load_policy(self.handle, C.byref(v)))
         return bool(v.value)
 
     def set_streaming(self, mode):

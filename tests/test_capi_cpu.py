@@ -75,6 +75,7 @@ def test_error_paths_do_not_throw():
     assert N.lib().pvlbm_set_streaming(None, N.STREAMING_TWO_LATTICES) == 1
     v = C.c_int(7)
     assert N.lib().pvlbm_get_streaming(None, C.byref(v)) == 1 and v.value == 7
+    assert N.lib().pvlbm_get_load_policy(None, C.byref(v)) == 1 and v.value == 7
     assert N.lib().pvlbm_init_fields_coarse(None, None, None, None, 0.3, 0.8, 1.25, 0.0, None, None, None) == 1
 
 

@@ -146,6 +146,7 @@ def test_store_policy_rule_and_identical_results():
                 sim.set_store_policy(policy)
                 expect = {N.STORES_AUTO: auto, N.STORES_REGULAR: False, N.STORES_STREAMING: True}[policy]
                 assert sim.streaming_stores() == expect, (storage, scheme, policy)
+                assert not sim.streaming_loads(), (storage, scheme, policy)  # outside the 180-240 MiB band
                 sim.step(41)
                 fields[(storage, scheme, policy)] = sim.macros() + (np.asarray(sim.reduce()),)
                 del sim, sd
@@ -155,3 +156,35 @@ def test_store_policy_rule_and_identical_results():
                 assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), key
                 assert np.array_equal(a[2], b[2]), (key, a[2], b[2])
     assert np.abs(fields[("fp64", N.STREAMING_IN_PLACE, N.STORES_AUTO)][1]).max() > 1e-3
+
+
+def test_nontemporal_loads_band_and_identical_results():
+    """In place with plain stores, the automatic policy reads the populations with nontemporal
+    loads once the lattice's lines that hold nodes exceed 180 MiB (and stay within the 240 MiB of
+    plain stores): a 128 x 128 x 88 all-fluid box, D3Q19 FP64, 219 MB.  Forced plain stores read
+    with plain loads; the pull never uses them.  Fields and FP64 global sums are the same bit for
+    bit under each, over 41 steps."""
+    from pv_dec_original_amd.simulation import Simulation, SubDomain
+    case = fluid_box_case(q=19, box=(128, 128, 88), tau=0.7)
+    n = 128 * 128 * 88
+    assert (180 << 20) < 19 * 8 * n <= (240 << 20)
+    fields = {}
+    for scheme, policy, loads in ((N.STREAMING_IN_PLACE, N.STORES_AUTO, True),
+                                  (N.STREAMING_IN_PLACE, N.STORES_REGULAR, False),
+                                  (N.STREAMING_TWO_LATTICES, N.STORES_AUTO, False)):
+        sd = SubDomain(case.geometry, case.faces, storage="fp64", tau=case.tau, dt=G.DT, triangles=False)
+        sim = Simulation(sd, case.tau, 1.0, G.DT)
+        sim.set_inlet_profile(case.inlet_profile(sd, 0.05))
+        sim.set_streaming(scheme)
+        sim.set_store_policy(policy)
+        assert sim.streaming_loads() == loads, (scheme, policy)
+        if scheme == N.STREAMING_IN_PLACE:
+            assert not sim.streaming_stores()
+        sim.step(41)
+        fields[(scheme, policy)] = sim.macros() + (np.asarray(sim.reduce()),)
+        del sim, sd
+    a = fields[(N.STREAMING_IN_PLACE, N.STORES_AUTO)]
+    for key, b in fields.items():
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), key
+        assert np.array_equal(a[2], b[2]), (key, a[2], b[2])
+    assert np.abs(a[1]).max() > 1e-3
