@@ -259,7 +259,8 @@ def main():
             "wss_accumulators": ("every iteration" if args.wss_every == 1 else
                                  f"sampled every {args.wss_every} iterations (OSI/TAWSS sums over the samples)"),
             "lists": counts, "population_stores": "nontemporal" if sim.streaming_stores() else "plain",
-            "population_loads": "nontemporal" if sim.streaming_loads() else "plain",
+            "population_loads": {0: "plain", 1: "nontemporal in the even launches",
+                                 2: "nontemporal"}[sim.streaming_loads()],
             "streaming": "in place, one lattice" if sim.in_place else "pull, two lattices",
             "wall_s_timed": round(wall, 3), "setup_s": round(t_setup, 2),
             "voxelise_s": round(t_geo, 2),

@@ -270,24 +270,24 @@ enum { PVLBM_STREAMING_AUTO = 0, PVLBM_STREAMING_TWO_LATTICES = 1, PVLBM_STREAMI
 int pvlbm_set_streaming(pvlbm_handle h, int mode);
 int pvlbm_get_streaming(pvlbm_handle h, int* in_place);
 
-/* How the step writes the populations (no counterpart in the reference; results are identical
- * under every policy).  PVLBM_STORES_AUTO (default): plain stores while the lattice the next
- * launch reads can stay in the MI355X's 256 MB last-level cache, so that launch reads it from
- * there, and nontemporal stores beyond, which keep the written populations from evicting the
- * lines the gathers are about to read: in place, nontemporal when the 128-byte lines of the one
- * lattice that hold nodes (lines per plane x 128 B x Q) exceed PVLBM_STORES_AUTO_INPLACE_BYTES
- * (240 MiB); with two lattices, when one step's traffic, 2 * Q * bytes per population * nodes,
- * exceeds PVLBM_STORES_AUTO_BYTES (128 MiB).  In place with plain stores, every other launch
- * sweeps the lattice in reverse, so that it starts on the lines the previous launch touched last.
- * Under PVLBM_STORES_AUTO the in-place step with plain stores also reads the populations with
- * nontemporal loads once those lines exceed PVLBM_LOADS_AUTO_INPLACE_BYTES (180 MiB), a lattice
- * that nearly fills the cache.  PVLBM_STORES_REGULAR / _STREAMING force one kind of store, with
- * plain loads.  pvlbm_get_store_policy reports the stores in use (1 = nontemporal),
- * pvlbm_get_load_policy the loads (1 = nontemporal). */
+/* How the step writes and reads the populations (no counterpart in the reference; results are
+ * identical under every policy).  PVLBM_STORES_AUTO (default), with two lattices: plain stores
+ * while one step's traffic, 2 * Q * bytes per population * nodes, stays within
+ * PVLBM_STORES_AUTO_BYTES (128 MiB), so that the next launch reads the lattice from the MI355X's
+ * 256 MB last-level cache, and nontemporal stores beyond, which keep the written populations
+ * from evicting the lines the gathers are about to read.  In place: plain stores, every other
+ * launch sweeping the lattice in reverse so that it starts on the lines the previous launch
+ * touched last, and nontemporal loads by the size of the lattice's lines that hold nodes (lines
+ * per plane x 128 B x Q): none up to PVLBM_LOADS_AUTO_INPLACE_BYTES (180 MiB), in every launch up
+ * to PVLBM_LOADS_AUTO_EVEN_BYTES (240 MiB), in the even launches (those that read and write each
+ * node's own slots) beyond.  PVLBM_STORES_REGULAR / _STREAMING force plain or nontemporal stores,
+ * with plain loads.  pvlbm_get_store_policy reports the stores in use (1 = nontemporal),
+ * pvlbm_get_load_policy the loads (0 = plain, 1 = nontemporal in the even launches, 2 = in every
+ * launch). */
 enum { PVLBM_STORES_AUTO = 0, PVLBM_STORES_REGULAR = 1, PVLBM_STORES_STREAMING = 2 };
 #define PVLBM_STORES_AUTO_BYTES (128ll << 20)
-#define PVLBM_STORES_AUTO_INPLACE_BYTES (240ll << 20)
 #define PVLBM_LOADS_AUTO_INPLACE_BYTES (180ll << 20)
+#define PVLBM_LOADS_AUTO_EVEN_BYTES (240ll << 20)
 int pvlbm_set_store_policy(pvlbm_handle h, int policy);
 int pvlbm_get_store_policy(pvlbm_handle h, int* streaming);
 int pvlbm_get_load_policy(pvlbm_handle h, int* streaming);

@@ -156,14 +156,16 @@ struct pvlbm_sim {
     size_t store_bytes() const { return storage == PVLBM_FP64 ? 8 : 2; }
     bool stream_stores() const {
         if (store_policy != PVLBM_STORES_AUTO) return store_policy == PVLBM_STORES_STREAMING;
-        if (in_place()) return footprint > PVLBM_STORES_AUTO_INPLACE_BYTES;
+        if (in_place()) return false;  // in place the loads carry the cache policy
         return (int64_t)(2 * q * store_bytes()) * nfluid > PVLBM_STORES_AUTO_BYTES;
     }
-    // nontemporal population loads: under the automatic policy, in place with plain stores, on a
-    // lattice whose lines holding nodes nearly fill the last-level cache
-    bool stream_loads() const {
-        return store_policy == PVLBM_STORES_AUTO && in_place() && !stream_stores() &&
-               footprint > PVLBM_LOADS_AUTO_INPLACE_BYTES;
+    // nontemporal population loads under the automatic policy, in place (plain stores), by the
+    // lines of the lattice that hold nodes: 2 = in every launch for a lattice that nearly fills
+    // the last-level cache, 1 = in the even launches only for a larger one, 0 = none
+    int stream_loads() const {
+        if (store_policy != PVLBM_STORES_AUTO || !in_place()) return 0;
+        if (footprint > PVLBM_LOADS_AUTO_EVEN_BYTES) return 1;
+        return footprint > PVLBM_LOADS_AUTO_INPLACE_BYTES ? 2 : 0;
     }
     size_t real_bytes() const { return storage == PVLBM_FP64 ? 8 : 4; }
     // in place on one lattice when asked, or by default while that lattice is within the cache
@@ -218,7 +220,6 @@ struct pvlbm_sim {
         p.blocks = blocks();
         p.flags = flags;
         p.stream_stores = stream_stores() ? 1 : 0;
-        p.stream_loads = stream_loads() ? 1 : 0;
         p.bmask = d_bmask;
         p.bclose = d_bclose;
         p.bprof = d_bprof;
@@ -286,6 +287,8 @@ struct pvlbm_sim {
             // launches sweep it backwards, so each launch starts on the lines the previous one
             // touched last
             p.reverse = (p.scheme == SCH_ODD && !p.stream_stores) ? 1 : 0;
+            const int loads = stream_loads();
+            p.stream_loads = (loads == 2 || (loads == 1 && p.scheme == SCH_EVEN)) ? 1 : 0;
             HIPCHK(launch_step(q, storage, p, stream));
             layout = p.scheme;
         } else {
@@ -1281,7 +1284,7 @@ extern "C" int pvlbm_get_load_policy(pvlbm_handle h, int* streaming) {
     if (!h) return PVLBM_EINVAL;
     return guarded(h, [&] {
         if (!streaming) fail(PVLBM_EINVAL, "streaming is NULL");
-        *streaming = h->stream_loads() ? 1 : 0;
+        *streaming = h->stream_loads();
     });
 }
 

@@ -197,9 +197,9 @@ class Simulation:
         self._chk(N.lib().pvlbm_set_wall_closure(self.handle, int(mode)))
 
     def set_store_policy(self, policy):
-        """N.STORES_AUTO (default: nontemporal population stores when the lattice the next launch
-        reads exceeds the cache budget: in place its lines holding nodes > 240 MiB, two lattices
-        2*Q*S*n > 128 MiB; in place with plain stores, nontemporal loads above 180 MiB of lines),
+        """N.STORES_AUTO (default: two lattices, nontemporal population stores when a step moves
+        2*Q*S*n > 128 MiB; in place, plain stores and, by the lattice's lines holding nodes,
+        nontemporal loads in every launch above 180 MiB, in the even launches above 240 MiB),
         N.STORES_REGULAR or N.STORES_STREAMING (plain loads); results are the same under each."""
         self._chk(N.lib().pvlbm_set_store_policy(self.handle, int(policy)))
 
@@ -210,10 +210,11 @@ class Simulation:
         return bool(v.value)
 
     def streaming_loads(self):
-        """True when the step reads the populations with nontemporal loads."""
+        """0: the step reads the populations with plain loads; 1: nontemporal loads in the even
+        in-place launches; 2: in every launch."""
         v = C.c_int(0)
         self._chk(N.lib().pvlbm_get_load_policy(self.handle, C.byref(v)))
-        return bool(v.value)
+        return int(v.value)
 
     def set_streaming(self, mode):
         """N.STREAMING_AUTO (in place on one lattice while it is at most 256 MiB, the pull from one

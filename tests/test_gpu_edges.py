@@ -121,14 +121,14 @@ def test_bad_arguments_are_errors_and_leave_the_state():
 
 
 def test_store_policy_rule_and_identical_results():
-    """The automatic store policy writes populations with nontemporal stores when the lattice the
-    next launch reads outgrows the last-level cache: in place when the lattice's lines that hold
-    nodes (Q S n for an all-fluid box of whole tiles) exceed 240 MiB, with two lattices when one
-    step's traffic 2 Q S n exceeds 128 MiB; plain stores below.  Forcing either policy, in either
-    scheme, gives the same fields bit for bit, and the same FP64 global sums (in place, plain
-    stores sweep every other launch in reverse block order; each block keeps its four tiles, so
-    its partial sums do not change).  A 128 x 128 x 104 all-fluid box, D3Q19: one
-    lattice of 259 MB in FP64 (nontemporal in both schemes) and 65 MB in FP16 (plain in both)."""
+    """The automatic policy with two lattices writes the populations with nontemporal stores when
+    one step's traffic 2 Q S n exceeds 128 MiB, plain stores below; in place it writes plain
+    stores and reads with nontemporal loads by the lattice's lines that hold nodes (Q S n for an
+    all-fluid box of whole tiles): in the even launches only above 240 MiB.  Forcing either store
+    policy (plain loads), in either scheme, gives the same fields bit for bit, and the same FP64
+    global sums (in place, every other launch sweeps in reverse block order; each block keeps its
+    four tiles, so its partial sums do not change).  A 128 x 128 x 104 all-fluid box, D3Q19: one
+    lattice of 259 MB in FP64 and 65 MB in FP16."""
     from pv_dec_original_amd.simulation import Simulation, SubDomain
     case = fluid_box_case(q=19, box=(128, 128, 104), tau=0.7)
     n = 128 * 128 * 104
@@ -136,17 +136,20 @@ def test_store_policy_rule_and_identical_results():
     for storage in ("fp64", "fp16"):
         S = 8 if storage == "fp64" else 2
         for scheme in (N.STREAMING_IN_PLACE, N.STREAMING_TWO_LATTICES):
-            auto = (19 * S * n > (240 << 20)) if scheme == N.STREAMING_IN_PLACE else (2 * 19 * S * n > (128 << 20))
-            assert auto == (storage == "fp64")
+            if scheme == N.STREAMING_IN_PLACE:
+                auto_stores, auto_loads = False, (1 if 19 * S * n > (240 << 20) else 0)
+            else:
+                auto_stores, auto_loads = 2 * 19 * S * n > (128 << 20), 0
+            assert auto_loads == (1 if storage == "fp64" and scheme == N.STREAMING_IN_PLACE else 0)
             for policy in (N.STORES_AUTO, N.STORES_REGULAR, N.STORES_STREAMING):
                 sd = SubDomain(case.geometry, case.faces, storage=storage, tau=case.tau, dt=G.DT, triangles=False)
                 sim = Simulation(sd, case.tau, 1.0, G.DT)
                 sim.set_inlet_profile(case.inlet_profile(sd, 0.05))
                 sim.set_streaming(scheme)
                 sim.set_store_policy(policy)
-                expect = {N.STORES_AUTO: auto, N.STORES_REGULAR: False, N.STORES_STREAMING: True}[policy]
+                expect = {N.STORES_AUTO: auto_stores, N.STORES_REGULAR: False, N.STORES_STREAMING: True}[policy]
                 assert sim.streaming_stores() == expect, (storage, scheme, policy)
-                assert not sim.streaming_loads(), (storage, scheme, policy)  # outside the 180-240 MiB band
+                assert sim.streaming_loads() == (auto_loads if policy == N.STORES_AUTO else 0), (storage, scheme, policy)
                 sim.step(41)
                 fields[(storage, scheme, policy)] = sim.macros() + (np.asarray(sim.reduce()),)
                 del sim, sd
@@ -159,19 +162,18 @@ def test_store_policy_rule_and_identical_results():
 
 
 def test_nontemporal_loads_band_and_identical_results():
-    """In place with plain stores, the automatic policy reads the populations with nontemporal
-    loads once the lattice's lines that hold nodes exceed 180 MiB (and stay within the 240 MiB of
-    plain stores): a 128 x 128 x 88 all-fluid box, D3Q19 FP64, 219 MB.  Forced plain stores read
-    with plain loads; the pull never uses them.  Fields and FP64 global sums are the same bit for
-    bit under each, over 41 steps."""
+    """In place, the automatic policy reads the populations with nontemporal loads in every launch
+    once the lattice's lines that hold nodes exceed 180 MiB (up to 240 MiB): a 128 x 128 x 88
+    all-fluid box, D3Q19 FP64, 219 MB.  Forced plain stores read with plain loads; the pull never
+    uses them.  Fields and FP64 global sums are the same bit for bit under each, over 41 steps."""
     from pv_dec_original_amd.simulation import Simulation, SubDomain
     case = fluid_box_case(q=19, box=(128, 128, 88), tau=0.7)
     n = 128 * 128 * 88
     assert (180 << 20) < 19 * 8 * n <= (240 << 20)
     fields = {}
-    for scheme, policy, loads in ((N.STREAMING_IN_PLACE, N.STORES_AUTO, True),
-                                  (N.STREAMING_IN_PLACE, N.STORES_REGULAR, False),
-                                  (N.STREAMING_TWO_LATTICES, N.STORES_AUTO, False)):
+    for scheme, policy, loads in ((N.STREAMING_IN_PLACE, N.STORES_AUTO, 2),
+                                  (N.STREAMING_IN_PLACE, N.STORES_REGULAR, 0),
+                                  (N.STREAMING_TWO_LATTICES, N.STORES_AUTO, 0)):
         sd = SubDomain(case.geometry, case.faces, storage="fp64", tau=case.tau, dt=G.DT, triangles=False)
         sim = Simulation(sd, case.tau, 1.0, G.DT)
         sim.set_inlet_profile(case.inlet_profile(sd, 0.05))
