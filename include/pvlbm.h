@@ -258,7 +258,7 @@ int pvlbm_set_wall_closure(pvlbm_handle h, int mode);
  * lattice into another but touches half the memory, so a lattice near the size of the MI355X's
  * 256 MB last-level cache is read from that cache; far beyond it the pull is faster.
  * PVLBM_STREAMING_AUTO (default): in place while the lattice, Q * bytes per population * nodes,
- * is at most PVLBM_STREAMING_AUTO_INPLACE_MAX_BYTES (256 MiB), the pull beyond.
+ * is at most PVLBM_STREAMING_AUTO_INPLACE_MAX_BYTES (512 MiB), the pull beyond.
  * PVLBM_STREAMING_TWO_LATTICES / _IN_PLACE force one (in place halves the population memory, for
  * domains too large for two lattices).  The interpolated wall closure (pvlbm_set_wall_closure)
  * reads the node's previous populations from its own slots and always takes the pull; it cannot
@@ -266,7 +266,7 @@ int pvlbm_set_wall_closure(pvlbm_handle h, int mode);
  * state into the new layout at once (the second lattice is allocated when first needed);
  * pvlbm_get_streaming reports the scheme in use (1 = in place). */
 enum { PVLBM_STREAMING_AUTO = 0, PVLBM_STREAMING_TWO_LATTICES = 1, PVLBM_STREAMING_IN_PLACE = 2 };
-#define PVLBM_STREAMING_AUTO_INPLACE_MAX_BYTES (256ll << 20)
+#define PVLBM_STREAMING_AUTO_INPLACE_MAX_BYTES (512ll << 20)
 int pvlbm_set_streaming(pvlbm_handle h, int mode);
 int pvlbm_get_streaming(pvlbm_handle h, int* in_place);
 

@@ -182,7 +182,7 @@ public:
     void setWssEvery(int every) { check(pvlbm_set_wss_every(h_, every), h_); }
     // PVLBM_WALL_REFERENCE (the reference's wall rule) or PVLBM_WALL_INTERPOLATED
     void setWallClosure(int mode) { check(pvlbm_set_wall_closure(h_, mode), h_); }
-    // PVLBM_STREAMING_AUTO (in place on one lattice up to 256 MiB, two lattices beyond),
+    // PVLBM_STREAMING_AUTO (in place on one lattice up to 512 MiB, two lattices beyond),
     // PVLBM_STREAMING_TWO_LATTICES or PVLBM_STREAMING_IN_PLACE; the fields are the same
     void setStreaming(int mode) { check(pvlbm_set_streaming(h_, mode), h_); }
     // restart from fields (rho [nz][ny][nx], u three such planes); empty vectors: from rest

@@ -217,7 +217,7 @@ class Simulation:
         return int(v.value)
 
     def set_streaming(self, mode):
-        """N.STREAMING_AUTO (in place on one lattice while it is at most 256 MiB, the pull from one
+        """N.STREAMING_AUTO (in place on one lattice while it is at most 512 MiB, the pull from one
         lattice into another beyond, and always the pull under the interpolated wall closure),
         N.STREAMING_TWO_LATTICES or N.STREAMING_IN_PLACE; same results under each."""
         self._chk(N.lib().pvlbm_set_streaming(self.handle, int(mode)))

@@ -5,7 +5,7 @@ neighbours' slots, links without an upstream node parked where the next launch r
 the pull from one lattice into another (pvlbm_set_streaming), over odd and even numbers of
 launches, with WSS and the OSI/TAWSS accumulators every iteration, for both lattices and the three
 storages; and across changes of scheme in the middle of a run from either kind of in-place launch,
-which move the state between the layouts.  By default a lattice of at most 256 MiB streams in place
+which move the state between the layouts.  By default a lattice of at most 512 MiB streams in place
 and a larger one takes the pull, so the small device parity tests against the oracle (push
 streaming, Simulation.cpp:211-237) run in place and the full-size FP64 aneurysm with two lattices.
 """
@@ -96,15 +96,15 @@ def test_warm_start_in_both_schemes():
 
 
 def test_automatic_scheme_by_lattice_size_and_forced_modes():
-    """A 128 x 128 x 112 all-fluid box, D3Q19: its FP64 lattice (279 MB) exceeds 256 MiB and takes
-    the pull by default, its FP16 lattice (70 MB) streams in place; forcing in place on the FP64
+    """A 128 x 128 x 224 all-fluid box, D3Q19: its FP64 lattice (558 MB) exceeds 512 MiB and takes
+    the pull by default, its FP16 lattice (139 MB) streams in place; forcing in place on the FP64
     box and running on gives the fields of a run that stayed on two lattices.  The interpolated
     wall closure cannot be combined with forced in-place streaming."""
     from pv_dec_original_amd.simulation import Simulation, SubDomain
     from pv_dec_original_amd.workloads import fluid_box_case
-    case = fluid_box_case(q=19, box=(128, 128, 112), tau=0.7)
-    n = 128 * 128 * 112
-    assert 19 * 8 * n > (256 << 20) >= 19 * 2 * n
+    case = fluid_box_case(q=19, box=(128, 128, 224), tau=0.7)
+    n = 128 * 128 * 224
+    assert 19 * 8 * n > (512 << 20) >= 19 * 2 * n
 
     def make(storage):
         sd = SubDomain(case.geometry, case.faces, storage=storage, tau=case.tau, dt=G.DT, triangles=False)
