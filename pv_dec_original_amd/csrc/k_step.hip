@@ -108,8 +108,8 @@ __device__ __forceinline__ S* at(P* base, int64_t sbytes, uint32_t lo) {
 }
 
 // All Q gathers issued back to back, before any store of the lane (in the in-place scheme a lane
-// writes the slots it read).  NL: nontemporal loads (in place, for a lattice that nearly fills the
-// last-level cache).
+// writes the slots it read).  NL: nontemporal loads (in place, for a lattice near or beyond the
+// size of the last-level cache).
 template <int Q, class ST, int SCH, bool NL, int D = 0>
 __device__ __forceinline__ void load_all(const StepParams& p, int64_t tb, uint32_t lo, typename ST::S (&v)[Q]) {
     if constexpr (D < Q) {
@@ -460,7 +460,7 @@ void launch_qws(const StepParams& p, hipStream_t s) {
         if (p.flags & F_WSS) hipLaunchKernelGGL((k_step<Q, ST, true, true, WI, SCH, false>), g, b, 0, s, p);
         else hipLaunchKernelGGL((k_step<Q, ST, false, true, WI, SCH, false>), g, b, 0, s, p);
     } else if constexpr (SCH != SCH_PULL) {
-        // in place on a lattice that nearly fills the last-level cache: nontemporal loads
+        // in place, nontemporal loads when the host asks for them (by the lattice's size)
         if (p.stream_loads) {
             if (p.flags & F_WSS) hipLaunchKernelGGL((k_step<Q, ST, true, false, WI, SCH, true>), g, b, 0, s, p);
             else hipLaunchKernelGGL((k_step<Q, ST, false, false, WI, SCH, true>), g, b, 0, s, p);
